@@ -1,0 +1,331 @@
+"""fmms_amd — MI355X (gfx950) FMM force evaluator, Python face of libfmmh.
+
+Mirrors the reference solver API `crates/fmm` (Vedant9500/FMMS):
+
+    Fmm(pos, mass, p, eps, n_leaf)          Fmm::new            fmm.rs:26-33
+    Fmm.with_center(..., mode)              Fmm::with_center    fmm.rs:36-49
+    fmm.refresh(pos, mass)                  Fmm::refresh        fmm.rs:54-56
+    fmm.run(pos, mass) -> Result            Fmm::run            fmm.rs:60-62
+    fmm.run_threaded(pos, mass, nthreads)   Fmm::run_threaded   fmm.rs:65-67
+    Result(accel [n,3], pot [n], n_cells)   fmm::Result         fmm.rs:17-21
+    plummer / hernquist                     models.rs:35-82
+
+A particle list `&[Particle]` becomes two arrays: pos [n, 3] and mass [n]
+(float64, particle-id order). Everything runs on the GPU through the C ABI
+declared in include/fmmh.h; there is no CPU fallback. If libfmmh.so is
+missing or cannot be loaded, importing this package raises.
+"""
+import ctypes as ct
+import math
+import os
+from collections import namedtuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmmh.so")
+
+FMMH_OK = 0
+FMMH_E_EMPTY = 1
+FMMH_E_PARAM = 2
+FMMH_E_OVER_BUDGET = 3
+FMMH_E_OVERFULL_LEAF = 4
+FMMH_E_NONFINITE = 5
+FMMH_E_HIP = 6
+FMMH_E_STATE = 7
+FMMH_FLAG_TIMING = 1
+MAX_P = 12
+
+_CODES = {1: "EMPTY", 2: "PARAM", 3: "OVER_BUDGET", 4: "OVERFULL_LEAF", 5: "NONFINITE", 6: "HIP",
+          7: "STATE"}
+
+
+class CenterMode:
+    """tree.rs:19-25"""
+    Com = 0
+    Sphere = 1
+
+
+class FmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fmmh error {_CODES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Params(ct.Structure):
+    _fields_ = [("p", ct.c_int), ("eps_mac", ct.c_double), ("theta_max", ct.c_double),
+                ("n_leaf", ct.c_int), ("softening", ct.c_double), ("center_mode", ct.c_int),
+                ("precision", ct.c_int), ("flags", ct.c_uint), ("hbm_budget_bytes", ct.c_double)]
+
+
+_pd = ct.POINTER(ct.c_double)
+_pu32 = ct.POINTER(ct.c_uint32)
+_pi32 = ct.POINTER(ct.c_int32)
+_pu64 = ct.POINTER(ct.c_uint64)
+_pu8 = ct.POINTER(ct.c_uint8)
+_vp = ct.c_void_p
+_sz = ct.c_size_t
+
+# Exported entry points and their signatures (must match include/fmmh.h).
+SIGNATURES = {
+    "fmmh_default_params": (None, [ct.POINTER(_Params)]),
+    "fmmh_create": (ct.c_int, [ct.POINTER(_Params), ct.POINTER(_vp)]),
+    "fmmh_destroy": (None, [_vp]),
+    "fmmh_last_error": (ct.c_char_p, [_vp]),
+    "fmmh_build": (ct.c_int, [_vp, _sz, _pd, _pd]),
+    "fmmh_evaluate": (ct.c_int, [_vp, _sz, _pd, _pd, _pd, _pd]),
+    "fmmh_accel": (ct.c_int, [_vp, _sz, _pd, _pd, _pd, _pd]),
+    "fmmh_build_device": (ct.c_int, [_vp, _sz, _vp, _vp]),
+    "fmmh_evaluate_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
+    "fmmh_accel_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
+    "fmmh_num_cells": (_sz, [_vp]),
+    "fmmh_get_tree": (ct.c_int, [_vp, _pu32, _pu64, _pu32, _pu32, _pi32, _pu8, _pu32, _pd, _pd, _pd]),
+    "fmmh_get_list_sizes": (ct.c_int, [_vp, ct.POINTER(_sz), ct.POINTER(_sz)]),
+    "fmmh_get_lists": (ct.c_int, [_vp, _pu32, _pu32]),
+    "fmmh_get_moments": (ct.c_int, [_vp, _pd]),
+    "fmmh_get_locals": (ct.c_int, [_vp, _pd]),
+    "fmmh_get_pows": (ct.c_int, [_vp, _pd]),
+    "fmmh_get_f": (ct.c_int, [_vp, _pd, _pd]),
+    "fmmh_stage_times": (ct.c_int, [_vp, _pd, ct.c_int]),
+    "fmmh_stage_name": (ct.c_char_p, [ct.c_int]),
+    "fmmh_work_counts": (ct.c_int, [_vp, _pu64]),
+    "fmmh_direct_sampled": (ct.c_int, [_sz, _pd, _pd, ct.c_double, _sz, _pu32, _pd, _pd, _pd]),
+    "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
+    "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m fmms_amd.build`")
+    lib = ct.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+def _p(a, t=_pd):
+    return a.ctypes.data_as(t)
+
+
+Result = namedtuple("Result", ["accel", "pot", "n_cells"])
+Tree = namedtuple("Tree", ["order", "keys", "first", "count", "children", "leaf", "level", "cen",
+                           "mass", "rho"])
+
+
+def dim(p):
+    return (p + 1) * (p + 1)
+
+
+class Fmm:
+    """GPU FMM with the reference `Fmm` interface. Construction builds the
+    tree (Fmm::new); `run` evaluates accelerations and potentials.
+
+    Extensions beyond the reference (defaults reproduce it): theta_max
+    (opening-angle bound, 1.0 = theta < 1), softening (Plummer softening of
+    the near field, 0 = pure 1/r), eps = inf for the theta-only MAC."""
+
+    def __init__(self, pos, mass, p, eps, n_leaf, center_mode=CenterMode.Com, theta_max=1.0,
+                 softening=0.0, timing=False, hbm_budget_bytes=0.0):
+        prm = _Params()
+        _lib.fmmh_default_params(ct.byref(prm))
+        prm.p, prm.eps_mac, prm.theta_max, prm.n_leaf = int(p), float(eps), float(theta_max), int(n_leaf)
+        prm.softening, prm.center_mode = float(softening), int(center_mode)
+        prm.flags = FMMH_FLAG_TIMING if timing else 0
+        prm.hbm_budget_bytes = float(hbm_budget_bytes)
+        self.p, self.eps, self.n_leaf = int(p), float(eps), int(n_leaf)
+        self.theta_max, self.softening, self.center_mode = float(theta_max), float(softening), int(center_mode)
+        h = _vp()
+        rc = _lib.fmmh_create(ct.byref(prm), ct.byref(h))
+        if rc != FMMH_OK:
+            raise FmmError(rc, "invalid parameters")
+        self._h = h
+        self.n = None
+        if pos is not None:
+            self.refresh(pos, mass)
+
+    @classmethod
+    def with_center(cls, pos, mass, p, eps, n_leaf, mode, **kw):
+        return cls(pos, mass, p, eps, n_leaf, center_mode=mode, **kw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.fmmh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != FMMH_OK:
+            raise FmmError(rc, _lib.fmmh_last_error(self._h).decode())
+
+    def refresh(self, pos, mass):
+        mass = _f64(mass)
+        n = mass.shape[0]
+        pos = _f64(pos, (n, 3)) if n else _f64(pos)
+        self._check(_lib.fmmh_build(self._h, n, _p(pos), _p(mass)))
+        self.n = n
+
+    @property
+    def n_cells(self):
+        return int(_lib.fmmh_num_cells(self._h))
+
+    def run(self, pos, mass):
+        mass = _f64(mass)
+        n = mass.shape[0]
+        pos = _f64(pos, (n, 3))
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        self._check(_lib.fmmh_evaluate(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
+        return Result(acc, pot, self.n_cells)
+
+    def run_threaded(self, pos, mass, nthreads=1):
+        """Same as `run`; the device decides its own parallelism."""
+        return self.run(pos, mass)
+
+    def accel(self, pos, mass):
+        """Physics::accel (sim.rs:5-7): rebuild the tree and evaluate."""
+        mass = _f64(mass)
+        n = mass.shape[0]
+        pos = _f64(pos, (n, 3))
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        self._check(_lib.fmmh_accel(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
+        self.n = n
+        return Result(acc, pot, self.n_cells)
+
+    # ---- device-resident entry points (raw device pointers, e.g. torch .data_ptr())
+    def accel_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr=0):
+        self._check(_lib.fmmh_accel_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
+                                           _vp(pot_ptr) if pot_ptr else None))
+        self.n = n
+
+    def build_device(self, n, pos_ptr, mass_ptr):
+        self._check(_lib.fmmh_build_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr)))
+        self.n = n
+
+    def evaluate_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr=0):
+        self._check(_lib.fmmh_evaluate_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
+                                              _vp(pot_ptr) if pot_ptr else None))
+
+    # ---- parity views
+    def tree(self):
+        n, nc = self.n, self.n_cells
+        t = Tree(np.empty(n, np.uint32), np.empty(n, np.uint64), np.empty(nc, np.uint32),
+                 np.empty(nc, np.uint32), np.empty((nc, 8), np.int32), np.empty(nc, np.uint8),
+                 np.empty(nc, np.uint32), np.empty((nc, 3)), np.empty(nc), np.empty(nc))
+        self._check(_lib.fmmh_get_tree(self._h, _p(t.order, _pu32), _p(t.keys, _pu64),
+                                       _p(t.first, _pu32), _p(t.count, _pu32),
+                                       _p(t.children, _pi32), _p(t.leaf, _pu8),
+                                       _p(t.level, _pu32), _p(t.cen), _p(t.mass), _p(t.rho)))
+        return t
+
+    def lists(self):
+        a, b = _sz(), _sz()
+        self._check(_lib.fmmh_get_list_sizes(self._h, ct.byref(a), ct.byref(b)))
+        m2l = np.empty((a.value, 2), np.uint32)
+        p2p = np.empty((b.value, 2), np.uint32)
+        self._check(_lib.fmmh_get_lists(self._h, _p(m2l, _pu32), _p(p2p, _pu32)))
+        return m2l, p2p
+
+    def moments(self):
+        out = np.empty((self.n_cells, dim(self.p)), np.complex128)
+        self._check(_lib.fmmh_get_moments(self._h, out.ctypes.data_as(_pd)))
+        return out
+
+    def locals(self):
+        out = np.empty((self.n_cells, dim(self.p)), np.complex128)
+        self._check(_lib.fmmh_get_locals(self._h, out.ctypes.data_as(_pd)))
+        return out
+
+    def pows(self):
+        out = np.empty((self.n_cells, self.p + 1))
+        self._check(_lib.fmmh_get_pows(self._h, _p(out)))
+        return out
+
+    def f(self):
+        f = np.empty(self.n)
+        fmin = np.empty(self.n_cells)
+        self._check(_lib.fmmh_get_f(self._h, _p(f), _p(fmin)))
+        return f, fmin
+
+    def stage_times(self):
+        buf = (ct.c_double * 32)()
+        k = _lib.fmmh_stage_times(self._h, buf, 32)
+        return {_lib.fmmh_stage_name(i).decode(): buf[i] for i in range(k)}
+
+    def work_counts(self):
+        out = np.zeros(8, np.uint64)
+        self._check(_lib.fmmh_work_counts(self._h, _p(out, _pu64)))
+        keys = ["n", "ncells", "nleaves", "m2l_pairs", "p2p_pairs", "p2p_interactions",
+                "max_level", "walk_rounds"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+
+# ------------------------------------------------------------------ models.rs
+def plummer(n, center=(0.0, 0.0, 0.0), seed=42):
+    pos = np.empty((n, 3))
+    mass = np.empty(n)
+    c = _f64(center)
+    _lib.fmmh_plummer(n, _p(c), seed, _p(pos), _p(mass))
+    return pos, mass
+
+
+def hernquist(n, center=(0.0, 0.0, 0.0), seed=7):
+    pos = np.empty((n, 3))
+    mass = np.empty(n)
+    c = _f64(center)
+    _lib.fmmh_hernquist(n, _p(c), seed, _p(pos), _p(mass))
+    return pos, mass
+
+
+def sample_ids(n, samp, seed=7):
+    """The bench's LCG target picker (crates/bench/src/main.rs:41-47)."""
+    out = np.empty(samp, np.uint32)
+    s = seed
+    mask = (1 << 64) - 1
+    for i in range(samp):
+        s = (s * 6364136223846793005 + 1442695040888963407) & mask
+        out[i] = (s >> 33) % n
+    return out
+
+
+def direct_sampled(pos, mass, ids, softening=0.0):
+    """FP64 direct sum on the GPU at target ids: (acc [k,3], pot [k], f [k])."""
+    mass = _f64(mass)
+    n = mass.shape[0]
+    pos = _f64(pos, (n, 3))
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    k = ids.shape[0]
+    acc = np.empty((k, 3))
+    pot = np.empty(k)
+    f = np.empty(k)
+    rc = _lib.fmmh_direct_sampled(n, _p(pos), _p(mass), float(softening), k, _p(ids, _pu32),
+                                  _p(acc), _p(pot), _p(f))
+    if rc != FMMH_OK:
+        raise FmmError(rc, "direct_sampled failed")
+    return acc, pot, f
+
+
+def rel_errors(acc, acc_ref):
+    """Per-target relative force error |a - a_ref| / |a_ref|."""
+    da = np.sqrt(np.sum((acc - acc_ref) ** 2, axis=1))
+    ar = np.sqrt(np.sum(acc_ref ** 2, axis=1))
+    return da / np.where(ar > 0, ar, 1.0)

@@ -1,0 +1,244 @@
+// downward.hip — L2L and L2P on gfx950 (FP64, FMA contraction on).
+//
+// L2L (translate.rs:175-206): F'_n^m(s') += sum_k sum_l conj(Y_k^l(s - s')) F_{n+k}^{m+l}(s)
+// L2P (translate.rs:271-309): Psi_n^m(x) = sum_k sum_l conj(Y_k^l(s - x)) F_{n+k}^{m+l}(s),
+//     pot += Re Psi_0^0, a -= (Re Psi_1^1, Im Psi_1^1, Re Psi_1^0).
+// Upsilon is streamed degree by degree (harmonic.rs:141-173 recursion) and
+// only the m >= 0 halves of the expansions are held (registers, P static).
+#include "common.h"
+#include "internal.h"
+
+namespace fmmh {
+
+__device__ __forceinline__ void cmac2(cplx& f, double ar, double ai, double br, double bi) {
+  f.re = fma(ar, br, f.re);
+  f.re = fma(-ai, bi, f.re);
+  f.im = fma(ar, bi, f.im);
+  f.im = fma(ai, br, f.im);
+}
+
+// Upsilon_k^q for q = 0..k from the two previous degrees (U1 = k-1, U2 = k-2).
+template <int P>
+__device__ __forceinline__ void upsilon_degree(int k, double x, double y, double z, double r2,
+                                               cplx* U, const cplx* U1, const cplx* U2) {
+  if (k == 0) {
+    U[0] = cplx{1.0, 0.0};
+    return;
+  }
+  const double a = (double)(2 * k - 1) * z;
+#pragma unroll
+  for (int q = 0; q <= P; q++) {
+    if (q > k) continue;
+    if (q == k) {  // (x+iy)/(2k) * Upsilon_{k-1}^{k-1}
+      const double f = 1.0 / (double)(2 * k);
+      const double wr = x * f, wi = y * f;
+      U[q].re = wr * U1[k - 1].re - wi * U1[k - 1].im;
+      U[q].im = wr * U1[k - 1].im + wi * U1[k - 1].re;
+    } else {
+      const double ic = 1.0 / (double)(k * k - q * q);
+      if (q + 2 <= k) {
+        U[q].re = (a * U1[q].re - r2 * U2[q].re) * ic;
+        U[q].im = (a * U1[q].im - r2 * U2[q].im) * ic;
+      } else {
+        U[q].re = a * U1[q].re * ic;
+        U[q].im = a * U1[q].im * ic;
+      }
+    }
+  }
+}
+
+// conj(Y_k^l) from the m>=0 column U (Y_k^{-l} = (-1)^l conj(Y_k^l)).
+__device__ __forceinline__ void conj_y(const cplx* U, int l, double* ar, double* ai) {
+  if (l >= 0) {
+    *ar = U[l].re;
+    *ai = -U[l].im;
+  } else {
+    const double sg = ((-l) & 1) ? -1.0 : 1.0;
+    *ar = sg * U[-l].re;
+    *ai = sg * U[-l].im;
+  }
+}
+
+// F_s^q from the m>=0 half (F_s^{-q} = (-1)^q conj(F_s^q)).
+template <int P>
+__device__ __forceinline__ void f_at(const cplx* F, int s, int q, double* br, double* bi) {
+  if (q >= 0) {
+    *br = F[hpos(s, q)].re;
+    *bi = F[hpos(s, q)].im;
+  } else {
+    const double sg = ((-q) & 1) ? -1.0 : 1.0;
+    *br = sg * F[hpos(s, -q)].re;
+    *bi = -sg * F[hpos(s, -q)].im;
+  }
+}
+
+// L2L, one thread per child cell of a level: locals[child] += shift(parent).
+template <int P>
+__global__ void __launch_bounds__(64) k_l2l(const uint32_t* __restrict__ cells, uint32_t cnt,
+                                           const int32_t* __restrict__ parent,
+                                           const double* __restrict__ cx,
+                                           const double* __restrict__ cy,
+                                           const double* __restrict__ cz,
+                                           cplx* __restrict__ locals) {
+  constexpr int D = hdim(P), NF = hnpos(P);
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint32_t c = cells[i];
+  const int32_t pa = parent[c];
+  if (pa < 0) return;
+  const double x = cx[pa] - cx[c], y = cy[pa] - cy[c], z = cz[pa] - cz[c];
+  const double r2 = x * x + y * y + z * z;
+  cplx Fp[NF], G[NF];
+  const cplx* lp = locals + (size_t)pa * D;
+  cplx* lc = locals + (size_t)c * D;
+#pragma unroll
+  for (int n = 0; n <= P; n++)
+#pragma unroll
+    for (int m = 0; m <= n; m++) {
+      Fp[hpos(n, m)] = lp[hidx(n, m)];
+      G[hpos(n, m)] = lc[hidx(n, m)];
+    }
+  cplx U[P + 1], U1[P + 1], U2[P + 1];
+#pragma unroll
+  for (int k = 0; k <= P; k++) {
+    upsilon_degree<P>(k, x, y, z, r2, U, U1, U2);
+#pragma unroll
+    for (int n = 0; n <= P; n++) {
+      if (n + k > P) continue;
+#pragma unroll
+      for (int m = 0; m <= P; m++) {
+        if (m > n) continue;
+#pragma unroll
+        for (int l = -P; l <= P; l++) {
+          if (l < -k || l > k) continue;
+          double ar, ai, br, bi;
+          conj_y(U, l, &ar, &ai);
+          f_at<P>(Fp, n + k, m + l, &br, &bi);
+          cmac2(G[hpos(n, m)], ar, ai, br, bi);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q <= P; q++) {
+      U2[q] = U1[q];
+      U1[q] = U[q];
+    }
+  }
+#pragma unroll
+  for (int n = 0; n <= P; n++)
+#pragma unroll
+    for (int m = 0; m <= n; m++) {
+      cplx g = G[hpos(n, m)];
+      lc[hidx(n, m)] = g;
+      if (m > 0) {
+        const double sg = (m & 1) ? -1.0 : 1.0;
+        lc[hidx(n, -m)] = cplx{sg * g.re, -sg * g.im};
+      }
+    }
+}
+
+// L2P, one thread per body (slot): far-field acceleration and potential.
+template <int P>
+__global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restrict__ leaf_of_slot,
+                                           const double* __restrict__ cx,
+                                           const double* __restrict__ cy,
+                                           const double* __restrict__ cz,
+                                           const double* __restrict__ bx,
+                                           const double* __restrict__ by,
+                                           const double* __restrict__ bz,
+                                           const cplx* __restrict__ locals,
+                                           double4* __restrict__ far) {
+  constexpr int D = hdim(P), NF = hnpos(P);
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t c = leaf_of_slot[k];
+  const double x = cx[c] - bx[k], y = cy[c] - by[k], z = cz[c] - bz[k];
+  const double r2 = x * x + y * y + z * z;
+  const cplx* lf = locals + (size_t)c * D;
+  cplx F[NF];
+#pragma unroll
+  for (int s = 0; s <= P; s++)
+#pragma unroll
+    for (int q = 0; q <= s; q++) F[hpos(s, q)] = lf[hidx(s, q)];
+  double p00 = 0.0, p11r = 0.0, p11i = 0.0, p10r = 0.0;
+  cplx U[P + 1], U1[P + 1], U2[P + 1];
+#pragma unroll
+  for (int kk = 0; kk <= P; kk++) {
+    upsilon_degree<P>(kk, x, y, z, r2, U, U1, U2);
+#pragma unroll
+    for (int l = -P; l <= P; l++) {
+      if (l < -kk || l > kk) continue;
+      double ar, ai, br, bi;
+      conj_y(U, l, &ar, &ai);
+      f_at<P>(F, kk, l, &br, &bi);
+      p00 = fma(ar, br, p00);
+      p00 = fma(-ai, bi, p00);
+      if (kk + 1 <= P) {
+        f_at<P>(F, kk + 1, l + 1, &br, &bi);
+        p11r = fma(ar, br, p11r);
+        p11r = fma(-ai, bi, p11r);
+        p11i = fma(ar, bi, p11i);
+        p11i = fma(ai, br, p11i);
+        f_at<P>(F, kk + 1, l, &br, &bi);
+        p10r = fma(ar, br, p10r);
+        p10r = fma(-ai, bi, p10r);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q <= P; q++) {
+      U2[q] = U1[q];
+      U1[q] = U[q];
+    }
+  }
+  far[k] = make_double4(-p11r, -p11i, -p10r, p00);
+}
+
+template <int P>
+static void l2l_t(const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2l<P>, dim3((cnt + 63) / 64), dim3(64), 0, s, cells, cnt, c.parent, c.cx,
+                     c.cy, c.cz, locals);
+}
+
+template <int P>
+static void l2p_t(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
+                  const cplx* locals, double4* far, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2p<P>, dim3((n + 63) / 64), dim3(64), 0, s, n, leaf_of_slot, c.cx, c.cy,
+                     c.cz, b.x, b.y, b.z, locals, far);
+}
+
+#define FMMH_P_SWITCH2(p, CALL) \
+  switch (p) {                  \
+    case 1: CALL(1); break;     \
+    case 2: CALL(2); break;     \
+    case 3: CALL(3); break;     \
+    case 4: CALL(4); break;     \
+    case 5: CALL(5); break;     \
+    case 6: CALL(6); break;     \
+    case 7: CALL(7); break;     \
+    case 8: CALL(8); break;     \
+    case 9: CALL(9); break;     \
+    case 10: CALL(10); break;   \
+    case 11: CALL(11); break;   \
+    case 12: CALL(12); break;   \
+    default: return false;      \
+  }
+
+bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
+                hipStream_t s) {
+  if (cnt == 0) return true;
+#define CALL(P) l2l_t<P>(cells, cnt, c, locals, s)
+  FMMH_P_SWITCH2(p, CALL)
+#undef CALL
+  return true;
+}
+
+bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
+                const cplx* locals, double4* far, hipStream_t s) {
+  if (n == 0) return true;
+#define CALL(P) l2p_t<P>(n, leaf_of_slot, c, b, locals, far, s)
+  FMMH_P_SWITCH2(p, CALL)
+#undef CALL
+  return true;
+}
+
+}  // namespace fmmh

@@ -1,0 +1,850 @@
+// fmmh.hip — host orchestration and the C ABI (include/fmmh.h).
+//
+// One context owns a HIP stream (default priority) and a device arena. A
+// build runs the Morton tree on the device (tree.hip); an evaluation runs
+// the upward pass, the optional f pre-pass, the dual walk, the target-grouped
+// M2L / L2L / L2P and the fused P2P + scatter. Sizes that decide launch
+// shapes (cells per level, walk frontier, list lengths) are read back once
+// per level / round; everything else stays on the device.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace fmmh;
+
+namespace {
+
+enum Stage {
+  ST_TREE = 0,
+  ST_UPWARD,
+  ST_PREPASS,
+  ST_WALK,
+  ST_LISTS,
+  ST_M2L,
+  ST_L2L,
+  ST_L2P,
+  ST_P2P,
+  ST_COUNT
+};
+const char* kStageNames[ST_COUNT] = {"tree",  "upward", "prepass", "walk", "lists",
+                                     "m2l",   "l2l",    "l2p",     "p2p"};
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct HipError {
+  std::string msg;
+};
+
+#define HCHECK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw HipError{std::string(#call) + ": " + hipGetErrorString(e_)};         \
+  } while (0)
+
+struct Budget {
+  size_t limit;
+};
+
+}  // namespace
+
+struct fmmh_ctx {
+  fmmh_params prm;
+  hipStream_t s = nullptr;
+  std::string err;
+  size_t used = 0;
+  std::vector<Buf*> bufs;
+  // state
+  bool built = false, evaluated = false;
+  size_t n = 0;
+  uint32_t nc = 0, nleaves = 0, max_level = 0;
+  int sbits = 1;
+  std::vector<uint32_t> lvl_off;
+  uint64_t* keys_sorted = nullptr;
+  uint32_t* order = nullptr;
+  uint32_t n_m2l = 0, n_p2p = 0, walk_rounds = 0;
+  uint64_t p2p_interactions = 0;
+  // timing
+  hipEvent_t ev[ST_COUNT + 1] = {};
+  bool ev_init = false;
+  double stage_ms[ST_COUNT] = {};
+  // buffers
+  Buf in_pos, in_mass, out_acc, out_pot;
+  Buf keys, keys_alt, vals, vals_alt;
+  Buf bx, by, bz, bm;
+  Buf bbox_partial, bbox, flags;
+  Buf chain, top, cstart;
+  Buf bfs_first, bfs_count, bfs_parent, bfs_oct, bfs_level, bfs_dfs, nchild, child_off;
+  Buf c_first, c_count, c_children, c_parent, c_leaf, c_level, c_cx, c_cy, c_cz, c_mass, c_rho,
+      c_rope, c_last;
+  Buf lvl_cells, leaf_flags, leaf_off, leaf_ids, leaf_of_slot;
+  Buf scan_tmp, sort_tmp;
+  Buf moments, locals, pows, fvals, fmin, fact, binom;
+  Buf front_a, front_b, m2l, p2p, ctr;
+  Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
+  Buf nch, choff, chunks, part;
+  Buf far;
+  Buf direct_ids, direct_out;
+  Buf pinned;
+
+  fmmh_ctx() {
+    Buf* all[] = {&in_pos, &in_mass, &out_acc, &out_pot, &keys, &keys_alt, &vals, &vals_alt,
+                  &bx, &by, &bz, &bm, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
+                  &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
+                  &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
+                  &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
+                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
+                  &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
+                  &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
+                  &pcount, &nch, &choff, &chunks, &part, &far, &direct_ids, &direct_out};
+    bufs.assign(std::begin(all), std::end(all));
+  }
+
+  ~fmmh_ctx() {
+    for (Buf* b : bufs)
+      if (b->p) (void)hipFree(b->p);
+    if (pinned.p) (void)hipHostFree(pinned.p);
+    if (ev_init)
+      for (auto& e : ev) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  // Grow a device buffer to at least `bytes` (content kept when keep=true).
+  template <class T = void>
+  T* ensure(Buf& b, size_t bytes, bool keep = false) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return (T*)b.p;
+    size_t want = std::max(bytes, b.cap + b.cap / 2);
+    size_t limit = (size_t)prm.hbm_budget_bytes;
+    if (limit && used - b.cap + want > limit) {
+      want = bytes;
+      if (used - b.cap + want > limit) throw Budget{limit};
+    }
+    void* np = nullptr;
+    HCHECK(hipMalloc(&np, want));
+    if (keep && b.p) HCHECK(hipMemcpyAsync(np, b.p, b.cap, hipMemcpyDeviceToDevice, s));
+    if (b.p) {
+      HCHECK(hipStreamSynchronize(s));
+      HCHECK(hipFree(b.p));
+    }
+    used = used - b.cap + want;
+    b.p = np;
+    b.cap = want;
+    return (T*)np;
+  }
+
+  void* host_stage(size_t bytes) {
+    if (pinned.cap < bytes) {
+      if (pinned.p) HCHECK(hipHostFree(pinned.p));
+      pinned.p = nullptr;
+      size_t want = std::max<size_t>(bytes, 4096);
+      HCHECK(hipHostMalloc(&pinned.p, want, hipHostMallocDefault));
+      pinned.cap = want;
+    }
+    return pinned.p;
+  }
+
+  template <class T>
+  T read1(const T* dev) {
+    T* h = (T*)host_stage(sizeof(T));
+    HCHECK(hipMemcpyAsync(h, dev, sizeof(T), hipMemcpyDeviceToHost, s));
+    HCHECK(hipStreamSynchronize(s));
+    return *h;
+  }
+
+  void mark(int st) {
+    if (!(prm.flags & FMMH_FLAG_TIMING)) return;
+    if (!ev_init) {
+      for (auto& e : ev) HCHECK(hipEventCreate(&e));
+      ev_init = true;
+    }
+    HCHECK(hipEventRecord(ev[st], s));
+  }
+
+  void finish_timing(int first, int last) {
+    if (!(prm.flags & FMMH_FLAG_TIMING) || !ev_init) return;
+    HCHECK(hipStreamSynchronize(s));
+    for (int st = first; st < last; st++) {
+      float ms = 0.f;
+      HCHECK(hipEventElapsedTime(&ms, ev[st], ev[st + 1]));
+      stage_ms[st] = ms;
+    }
+  }
+
+  DevCells cells() {
+    return DevCells{(uint32_t*)c_first.p, (uint32_t*)c_count.p, (int32_t*)c_children.p,
+                    (int32_t*)c_parent.p, (uint8_t*)c_leaf.p,  (uint8_t*)c_level.p,
+                    (double*)c_cx.p,      (double*)c_cy.p,     (double*)c_cz.p,
+                    (double*)c_mass.p,    (double*)c_rho.p,    (int32_t*)c_rope.p,
+                    (int32_t*)c_last.p};
+  }
+  DevBodies bodies() { return DevBodies{(double*)bx.p, (double*)by.p, (double*)bz.p, (double*)bm.p}; }
+  DevBfs bfs() {
+    return DevBfs{(uint32_t*)bfs_first.p, (uint32_t*)bfs_count.p, (int32_t*)bfs_parent.p,
+                  (uint8_t*)bfs_oct.p,    (uint8_t*)bfs_level.p,  (uint32_t*)bfs_dfs.p};
+  }
+
+  void ensure_bfs(size_t cap, bool keep) {
+    ensure(bfs_first, cap * 4, keep);
+    ensure(bfs_count, cap * 4, keep);
+    ensure(bfs_parent, cap * 4, keep);
+    ensure(bfs_oct, cap, keep);
+    ensure(bfs_level, cap, keep);
+    ensure(bfs_dfs, cap * 4, false);
+  }
+
+  void ensure_scan(size_t n_elems) {
+    ensure(scan_tmp, scan_tmp_words(n_elems) * 4 + 64);
+  }
+
+  // ------------------------------------------------------------------ build
+  void build(size_t n_, const double* pos, const double* mass) {
+    n = n_;
+    built = evaluated = false;
+    const int p = prm.p;
+    (void)p;
+    ensure(keys, n * 8);
+    ensure(keys_alt, n * 8);
+    ensure(vals, n * 4);
+    ensure(vals_alt, n * 4);
+    ensure(bx, n * 8);
+    ensure(by, n * 8);
+    ensure(bz, n * 8);
+    ensure(bm, n * 8);
+    ensure(bbox_partial, 6 * 1024 * 8);
+    ensure(bbox, sizeof(BBox));
+    ensure(flags, 64);
+    ensure(chain, n * 4);
+    ensure(top, n);
+    ensure(cstart, n * 4);
+    ensure(sort_tmp, radix_tmp_words(n) * 4 + 64);
+    ensure_scan(n);
+    mark(ST_TREE);
+    HCHECK(hipMemsetAsync(flags.p, 0, 64, s));
+    launch_bbox(pos, n, (double*)bbox_partial.p, (BBox*)bbox.p, s);
+    launch_keys(pos, mass, n, (const BBox*)bbox.p, (uint64_t*)keys.p, (uint32_t*)vals.p,
+                (uint32_t*)flags.p, s);
+    int flip = radix_sort_u64((uint64_t*)keys.p, (uint32_t*)vals.p, (uint64_t*)keys_alt.p,
+                              (uint32_t*)vals_alt.p, n, 0, 63, (uint32_t*)sort_tmp.p, s);
+    keys_sorted = (uint64_t*)(flip ? keys_alt.p : keys.p);
+    order = (uint32_t*)(flip ? vals_alt.p : vals.p);
+    launch_gather(pos, mass, order, n, bodies(), s);
+    // level-by-level octant splits
+    HCHECK(hipMemsetAsync(chain.p, 0, n * 4, s));
+    HCHECK(hipMemsetAsync(top.p, 0, n, s));
+    size_t cap = std::max<size_t>(1024, 4 * (n / std::max(1, prm.n_leaf)) + 64);
+    ensure_bfs(cap, false);
+    ensure(nchild, cap * 4);
+    ensure(child_off, cap * 4);
+    launch_tree_init(bfs(), n, (uint32_t*)chain.p, (uint8_t*)top.p, s);
+    if (read1((const uint32_t*)flags.p) != 0) throw int(FMMH_E_NONFINITE);
+    lvl_off.assign({0u, 1u});
+    for (int L = 0;; L++) {
+      uint32_t s0 = lvl_off[L], cnt = lvl_off[L + 1] - s0;
+      ensure(nchild, (size_t)cnt * 4);
+      ensure(child_off, (size_t)cnt * 4);
+      ensure_scan(cnt);
+      launch_split_count(bfs(), s0, cnt, keys_sorted, (uint32_t)prm.n_leaf, (uint32_t*)nchild.p, s);
+      uint32_t* tot = nullptr;
+      scan_exclusive_u32((uint32_t*)nchild.p, (uint32_t*)child_off.p, cnt, (uint32_t*)scan_tmp.p, s,
+                         &tot);
+      uint32_t total = read1(tot);
+      if (total == 0) break;
+      size_t need = (size_t)lvl_off[L + 1] + total;
+      if (need > bfs_first.cap / 4) ensure_bfs(need + need / 2, true);
+      launch_split_emit(bfs(), s0, cnt, lvl_off[L + 1], keys_sorted, (uint32_t)prm.n_leaf,
+                        (uint32_t*)child_off.p, (uint32_t*)chain.p, (uint8_t*)top.p, s);
+      lvl_off.push_back((uint32_t)need);
+    }
+    nc = lvl_off.back();
+    max_level = (uint32_t)lvl_off.size() - 2;
+    scan_exclusive_u32((uint32_t*)chain.p, (uint32_t*)cstart.p, n, (uint32_t*)scan_tmp.p, s, nullptr);
+    launch_dfs_ids(bfs(), nc, (uint32_t*)cstart.p, (uint8_t*)top.p, s);
+    ensure(c_first, nc * 4);
+    ensure(c_count, nc * 4);
+    ensure(c_children, (size_t)nc * 32);
+    ensure(c_parent, nc * 4);
+    ensure(c_leaf, nc);
+    ensure(c_level, nc);
+    ensure(c_cx, nc * 8);
+    ensure(c_cy, nc * 8);
+    ensure(c_cz, nc * 8);
+    ensure(c_mass, nc * 8);
+    ensure(c_rho, nc * 8);
+    ensure(c_rope, nc * 4);
+    ensure(c_last, nc * 4);
+    ensure(lvl_cells, nc * 4);
+    ensure(leaf_flags, nc * 4);
+    ensure(leaf_off, nc * 4);
+    ensure(leaf_ids, nc * 4);
+    ensure(leaf_of_slot, n * 4);
+    ensure_scan(nc);
+    HCHECK(hipMemsetAsync(c_children.p, 0xFF, (size_t)nc * 32, s));
+    launch_scatter_cells(bfs(), nc, (uint32_t)prm.n_leaf, cells(), (uint32_t*)lvl_cells.p, s);
+    for (int L = (int)max_level; L >= 0; L--)
+      launch_geometry((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(),
+                      bodies(), prm.center_mode, s);
+    for (int L = 0; L <= (int)max_level; L++)
+      launch_ropes((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(), s);
+    launch_leaf_flags(cells(), nc, (uint32_t*)leaf_flags.p, s);
+    uint32_t* tot = nullptr;
+    scan_exclusive_u32((uint32_t*)leaf_flags.p, (uint32_t*)leaf_off.p, nc, (uint32_t*)scan_tmp.p, s,
+                       &tot);
+    launch_leaf_compact(cells(), nc, (uint32_t*)leaf_off.p, (uint32_t*)leaf_ids.p,
+                        (uint32_t*)leaf_of_slot.p, s);
+    nleaves = read1(tot);
+    sbits = 1;
+    while ((1ull << sbits) <= nc) sbits++;
+    built = true;
+  }
+
+  // --------------------------------------------------------------- evaluate
+  void evaluate(const double* pos, const double* mass, double* acc, double* pot, bool regather) {
+    const int p = prm.p;
+    const size_t D = (size_t)hdim(p);
+    DevCells c = cells();
+    DevBodies b = bodies();
+    if (regather) {
+      HCHECK(hipMemsetAsync(flags.p, 0, 64, s));
+      launch_gather(pos, mass, order, n, b, s);
+    }
+    mark(ST_UPWARD);
+    cplx* mom = ensure<cplx>(moments, nc * D * sizeof(cplx));
+    cplx* loc = ensure<cplx>(locals, nc * D * sizeof(cplx));
+    launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
+    for (int L = (int)max_level - 1; L >= 0; L--)
+      launch_m2m(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom, s);
+    mark(ST_PREPASS);
+    const bool use_err = std::isfinite(prm.eps_mac);
+    {
+      std::vector<double> f(2 * p + 1, 1.0);
+      for (size_t i = 1; i < f.size(); i++) f[i] = f[i - 1] * (double)i;  // moments.rs:11-17
+      std::vector<double> bn(p + 1, 1.0);
+      for (int k = 0; k <= p; k++) bn[k] = f[p] / (f[k] * f[p - k]);  // moments.rs:20-27
+      double* hf = (double*)host_stage((f.size() + bn.size()) * 8);
+      std::memcpy(hf, f.data(), f.size() * 8);
+      std::memcpy(hf + f.size(), bn.data(), bn.size() * 8);
+      ensure(fact, f.size() * 8);
+      ensure(binom, bn.size() * 8);
+      HCHECK(hipMemcpyAsync(fact.p, hf, f.size() * 8, hipMemcpyHostToDevice, s));
+      HCHECK(hipMemcpyAsync(binom.p, hf + f.size(), bn.size() * 8, hipMemcpyHostToDevice, s));
+      HCHECK(hipStreamSynchronize(s));
+    }
+    ensure(fmin, nc * 8);
+    if (use_err) {
+      ensure(pows, (size_t)nc * (p + 1) * 8);
+      ensure(fvals, n * 8);
+      launch_pows(p, nc, mom, (double*)fact.p, (double*)pows.p, s);
+      launch_fprepass((uint32_t)n, nc, c, b, (double*)fvals.p, s);
+      for (int L = (int)max_level; L >= 0; L--)
+        launch_fmin((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c,
+                    (double*)fvals.p, (double*)fmin.p, s);
+    }
+    mark(ST_WALK);
+    walk(use_err);
+    mark(ST_LISTS);
+    // directed M2L entries grouped by target
+    const int kbits = 2 * sbits;
+    ensure(mkeys, (size_t)2 * n_m2l * 8);
+    ensure(mkeys_alt, (size_t)2 * n_m2l * 8);
+    ensure(tptr, ((size_t)nc + 1) * 4);
+    ensure(sort_tmp, radix_tmp_words((size_t)2 * n_m2l) * 4 + 64);
+    launch_directed_keys_m2l((uint2*)m2l.p, n_m2l, sbits, (uint64_t*)mkeys.p, s);
+    int fl = radix_sort_u64((uint64_t*)mkeys.p, nullptr, (uint64_t*)mkeys_alt.p, nullptr,
+                            (size_t)2 * n_m2l, 0, kbits, (uint32_t*)sort_tmp.p, s);
+    const uint64_t* msorted = (const uint64_t*)(fl ? mkeys_alt.p : mkeys.p);
+    launch_csr_ptr(msorted, 2 * n_m2l, sbits, nc, (uint32_t*)tptr.p, s);
+    // directed P2P entries grouped by target
+    ensure(pkeys, (size_t)2 * n_p2p * 8);
+    ensure(pkeys_alt, (size_t)2 * n_p2p * 8);
+    ensure(pptr, ((size_t)nc + 1) * 4);
+    ensure(pcount, 64);
+    ensure(sort_tmp, radix_tmp_words((size_t)2 * n_p2p) * 4 + 64);
+    HCHECK(hipMemsetAsync(pcount.p, 0, 4, s));
+    launch_directed_keys_p2p((uint2*)p2p.p, n_p2p, sbits, (uint64_t*)pkeys.p, (uint32_t*)pcount.p, s);
+    fl = radix_sort_u64((uint64_t*)pkeys.p, nullptr, (uint64_t*)pkeys_alt.p, nullptr,
+                        (size_t)2 * n_p2p, 0, kbits, (uint32_t*)sort_tmp.p, s);
+    const uint64_t* psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
+    uint32_t nself = read1((const uint32_t*)pcount.p);
+    uint32_t pnnz = 2 * n_p2p - nself;
+    launch_csr_ptr(psorted, pnnz, sbits, nc, (uint32_t*)pptr.p, s);
+    // M2L chunks
+    size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
+    ensure(nch, ((size_t)nc + 1) * 4);
+    ensure(choff, ((size_t)nc + 1) * 4);
+    ensure(chunks, maxck * m2l_chunk_bytes());
+    ensure_scan(nc + 1);
+    uint32_t* tot = nullptr;
+    m2l_build_chunks((uint32_t*)tptr.p, nc, (uint32_t*)nch.p, (uint32_t*)choff.p,
+                     (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
+    // choff[nc] = total chunks (the reduce reads choff[t+1])
+    HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
+    uint32_t nchunks = read1(tot);
+    ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
+    mark(ST_M2L);
+    launch_m2l_chunks(p, nc, (uint32_t*)choff.p, chunks.p, nchunks, msorted, sbits, c, mom,
+                      (cplx*)part.p, loc, s);
+    mark(ST_L2L);
+    for (int L = 1; L <= (int)max_level; L++)
+      launch_l2l(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, loc, s);
+    mark(ST_L2P);
+    ensure(far, n * sizeof(double4));
+    launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
+    mark(ST_P2P);
+    launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, sbits, c, b,
+               prm.softening * prm.softening, (double4*)far.p, order, acc, pot, s);
+    mark(ST_COUNT);
+    HCHECK(hipStreamSynchronize(s));
+    HCHECK(hipGetLastError());
+    evaluated = true;
+  }
+
+  void walk(bool use_err) {
+    const uint32_t cap0 = 4 * nc + 1024;
+    if (front_a.cap < (size_t)cap0 * 8) {
+      ensure(front_a, (size_t)cap0 * 8);
+      ensure(front_b, (size_t)cap0 * 8);
+    }
+    ensure(m2l, (size_t)(32 * (size_t)nc + 1024) * 8);
+    ensure(p2p, (size_t)(16 * (size_t)nc + 1024) * 8);
+    ensure(ctr, 64);
+    MacParams mp;
+    mp.p = prm.p;
+    mp.use_err = use_err ? 1 : 0;
+    mp.eps = prm.eps_mac;
+    mp.theta2 = prm.theta_max * prm.theta_max;
+    mp.pows = (const double*)pows.p;
+    mp.binom = (const double*)binom.p;
+    mp.fmin = (const double*)fmin.p;
+    uint2 root = make_uint2(0, 0);
+    HCHECK(hipMemcpyAsync(front_a.p, &root, 8, hipMemcpyHostToDevice, s));
+    uint32_t nf = 1;
+    n_m2l = n_p2p = 0;
+    walk_rounds = 0;
+    Buf* fin = &front_a;
+    Buf* fout = &front_b;
+    WalkCounters* hc = (WalkCounters*)host_stage(sizeof(WalkCounters));
+    while (nf > 0) {
+      WalkCounters init{0, n_m2l, n_p2p, 0};
+      *hc = init;
+      HCHECK(hipMemcpyAsync(ctr.p, hc, sizeof init, hipMemcpyHostToDevice, s));
+      launch_walk_round((const uint2*)fin->p, nf, (uint2*)fout->p, (uint32_t)(fout->cap / 8),
+                        (uint2*)m2l.p, (uint32_t)(m2l.cap / 8), (uint2*)p2p.p,
+                        (uint32_t)(p2p.cap / 8), (WalkCounters*)ctr.p, cells(), mp, s);
+      HCHECK(hipMemcpyAsync(hc, ctr.p, sizeof init, hipMemcpyDeviceToHost, s));
+      HCHECK(hipStreamSynchronize(s));
+      WalkCounters got = *hc;
+      if (got.overflow) {
+        if (got.next > fout->cap / 8) ensure(*fout, (size_t)got.next * 8 * 2);
+        if (got.m2l > m2l.cap / 8) ensure(m2l, (size_t)got.m2l * 8 * 2, true);
+        if (got.p2p > p2p.cap / 8) ensure(p2p, (size_t)got.p2p * 8 * 2, true);
+        continue;  // re-run the round: its input frontier is unchanged
+      }
+      n_m2l = got.m2l;
+      n_p2p = got.p2p;
+      nf = got.next;
+      std::swap(fin, fout);
+      walk_rounds++;
+      if (fin->cap > fout->cap) ensure(*fout, fin->cap);
+    }
+  }
+};
+
+namespace {
+
+int fail(fmmh_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int check_params(const fmmh_params* p, std::string* why) {
+  if (p->p < 1 || p->p > FMMH_MAX_P) {
+    *why = "p must be in [1, " + std::to_string(FMMH_MAX_P) + "]";
+    return FMMH_E_PARAM;
+  }
+  if (p->n_leaf < 1) {
+    *why = "n_leaf must be >= 1";
+    return FMMH_E_PARAM;
+  }
+  if (!(p->theta_max > 0.0 && p->theta_max <= 1.0)) {
+    *why = "theta_max must be in (0, 1]";
+    return FMMH_E_PARAM;
+  }
+  if (!(p->eps_mac > 0.0)) {
+    *why = "eps_mac must be > 0 (INFINITY disables the error test)";
+    return FMMH_E_PARAM;
+  }
+  if (!(p->softening >= 0.0) || !std::isfinite(p->softening)) {
+    *why = "softening must be finite and >= 0";
+    return FMMH_E_PARAM;
+  }
+  if (p->center_mode != FMMH_CENTER_COM && p->center_mode != FMMH_CENTER_SPHERE) {
+    *why = "center_mode must be FMMH_CENTER_COM or FMMH_CENTER_SPHERE";
+    return FMMH_E_PARAM;
+  }
+  if (p->precision != FMMH_PREC_FP64) {
+    *why = "only FMMH_PREC_FP64 is available";
+    return FMMH_E_PARAM;
+  }
+  return FMMH_OK;
+}
+
+template <class F>
+int guarded(fmmh_ctx* c, F fn) {
+  if (!c) return FMMH_E_STATE;
+  try {
+    fn();
+    return FMMH_OK;
+  } catch (const HipError& e) {
+    return fail(c, FMMH_E_HIP, e.msg);
+  } catch (const Budget& b) {
+    return fail(c, FMMH_E_OVER_BUDGET,
+                "device memory budget of " + std::to_string(b.limit) + " bytes exceeded");
+  } catch (int code) {
+    return fail(c, code, code == FMMH_E_NONFINITE ? "non-finite position or mass" : "error");
+  }
+}
+
+// models.rs:13-31 SplitMix64
+struct SplitMix {
+  uint64_t s;
+  uint64_t next() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double unit() { return (double)(next() >> 11) / (double)(1ull << 53); }
+};
+
+// sin and cos of one angle through glibc's sincos: the route LLVM (the
+// reference's Rust) and GCC take for a sin/cos pair of the same argument;
+// separate sin() and cos() calls can differ from it in the last bit.
+inline void sin_cos(double a, double* s, double* c) { ::sincos(a, s, c); }
+
+}  // namespace
+
+extern "C" {
+
+void fmmh_default_params(fmmh_params* out) {
+  out->p = 8;
+  out->eps_mac = 1e-5;
+  out->theta_max = 1.0;
+  out->n_leaf = 16;
+  out->softening = 0.0;
+  out->center_mode = FMMH_CENTER_COM;
+  out->precision = FMMH_PREC_FP64;
+  out->flags = 0;
+  out->hbm_budget_bytes = 0.0;
+}
+
+int fmmh_create(const fmmh_params* params, fmmh_ctx** out) {
+  if (!params || !out) return FMMH_E_PARAM;
+  *out = nullptr;
+  std::string why;
+  int rc = check_params(params, &why);
+  if (rc != FMMH_OK) return rc;
+  fmmh_ctx* c = new fmmh_ctx();
+  c->prm = *params;
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FMMH_E_HIP;
+  }
+  *out = c;
+  return FMMH_OK;
+}
+
+void fmmh_destroy(fmmh_ctx* ctx) { delete ctx; }
+
+const char* fmmh_last_error(const fmmh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fmmh_build_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass) {
+  if (!ctx) return FMMH_E_STATE;
+  if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
+  return guarded(ctx, [&] {
+    ctx->build(n, d_pos, d_mass);
+    ctx->finish_timing(ST_TREE, ST_TREE);
+  });
+}
+
+int fmmh_evaluate_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
+                         double* d_accel, double* d_pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!ctx->built) return fail(ctx, FMMH_E_STATE, "evaluate before build");
+  if (n != ctx->n) return fail(ctx, FMMH_E_STATE, "n differs from the built tree");
+  return guarded(ctx, [&] {
+    ctx->evaluate(d_pos, d_mass, d_accel, d_pot, true);
+    ctx->stage_ms[ST_TREE] = 0.0;
+    ctx->finish_timing(ST_UPWARD, ST_COUNT);
+  });
+}
+
+int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
+                      double* d_accel, double* d_pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
+  return guarded(ctx, [&] {
+    ctx->build(n, d_pos, d_mass);
+    ctx->evaluate(d_pos, d_mass, d_accel, d_pot, false);
+    ctx->finish_timing(ST_TREE, ST_COUNT);
+  });
+}
+
+static int upload(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass) {
+  return guarded(ctx, [&] {
+    ctx->ensure(ctx->in_pos, n * 24);
+    ctx->ensure(ctx->in_mass, n * 8);
+    HCHECK(hipMemcpyAsync(ctx->in_pos.p, pos, n * 24, hipMemcpyHostToDevice, ctx->s));
+    HCHECK(hipMemcpyAsync(ctx->in_mass.p, mass, n * 8, hipMemcpyHostToDevice, ctx->s));
+  });
+}
+
+static int download(fmmh_ctx* ctx, size_t n, double* accel, double* pot) {
+  return guarded(ctx, [&] {
+    HCHECK(hipMemcpyAsync(accel, ctx->out_acc.p, n * 24, hipMemcpyDeviceToHost, ctx->s));
+    if (pot) HCHECK(hipMemcpyAsync(pot, ctx->out_pot.p, n * 8, hipMemcpyDeviceToHost, ctx->s));
+    HCHECK(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int fmmh_build(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass) {
+  if (!ctx) return FMMH_E_STATE;
+  if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  int rc = upload(ctx, n, pos, mass);
+  if (rc) return rc;
+  return fmmh_build_device(ctx, n, (const double*)ctx->in_pos.p, (const double*)ctx->in_mass.p);
+}
+
+static int outputs(fmmh_ctx* ctx, size_t n) {
+  return guarded(ctx, [&] {
+    ctx->ensure(ctx->out_acc, n * 24);
+    ctx->ensure(ctx->out_pot, n * 8);
+  });
+}
+
+int fmmh_evaluate(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass, double* accel,
+                  double* pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!ctx->built) return fail(ctx, FMMH_E_STATE, "evaluate before build");
+  if (n != ctx->n) return fail(ctx, FMMH_E_STATE, "n differs from the built tree");
+  int rc = upload(ctx, n, pos, mass);
+  if (!rc) rc = outputs(ctx, n);
+  if (!rc)
+    rc = fmmh_evaluate_device(ctx, n, (const double*)ctx->in_pos.p, (const double*)ctx->in_mass.p,
+                              (double*)ctx->out_acc.p, (double*)ctx->out_pot.p);
+  if (!rc) rc = download(ctx, n, accel, pot);
+  return rc;
+}
+
+int fmmh_accel(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass, double* accel,
+               double* pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  int rc = upload(ctx, n, pos, mass);
+  if (!rc) rc = outputs(ctx, n);
+  if (!rc)
+    rc = fmmh_accel_device(ctx, n, (const double*)ctx->in_pos.p, (const double*)ctx->in_mass.p,
+                           (double*)ctx->out_acc.p, (double*)ctx->out_pot.p);
+  if (!rc) rc = download(ctx, n, accel, pot);
+  return rc;
+}
+
+size_t fmmh_num_cells(const fmmh_ctx* ctx) { return ctx && ctx->built ? ctx->nc : 0; }
+
+int fmmh_get_tree(fmmh_ctx* ctx, uint32_t* order, uint64_t* keys_sorted, uint32_t* first,
+                  uint32_t* count, int32_t* children, uint8_t* leaf, uint32_t* level, double* cen,
+                  double* mass, double* rho) {
+  if (!ctx || !ctx->built) return fail(ctx, FMMH_E_STATE, "no tree");
+  return guarded(ctx, [&] {
+    const size_t n = ctx->n, nc = ctx->nc;
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+      if (dst) HCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->s));
+    };
+    d2h(order, ctx->order, n * 4);
+    d2h(keys_sorted, ctx->keys_sorted, n * 8);
+    d2h(first, ctx->c_first.p, nc * 4);
+    d2h(count, ctx->c_count.p, nc * 4);
+    d2h(children, ctx->c_children.p, nc * 32);
+    d2h(leaf, ctx->c_leaf.p, nc);
+    d2h(mass, ctx->c_mass.p, nc * 8);
+    d2h(rho, ctx->c_rho.p, nc * 8);
+    std::vector<uint8_t> lv(nc);
+    std::vector<double> x(nc), y(nc), z(nc);
+    if (level) d2h(lv.data(), ctx->c_level.p, nc);
+    if (cen) {
+      d2h(x.data(), ctx->c_cx.p, nc * 8);
+      d2h(y.data(), ctx->c_cy.p, nc * 8);
+      d2h(z.data(), ctx->c_cz.p, nc * 8);
+    }
+    HCHECK(hipStreamSynchronize(ctx->s));
+    if (level)
+      for (size_t i = 0; i < nc; i++) level[i] = lv[i];
+    if (cen)
+      for (size_t i = 0; i < nc; i++) {
+        cen[3 * i] = x[i];
+        cen[3 * i + 1] = y[i];
+        cen[3 * i + 2] = z[i];
+      }
+  });
+}
+
+int fmmh_get_list_sizes(fmmh_ctx* ctx, size_t* n_m2l, size_t* n_p2p) {
+  if (!ctx || !ctx->evaluated) return fail(ctx, FMMH_E_STATE, "no evaluation");
+  if (n_m2l) *n_m2l = ctx->n_m2l;
+  if (n_p2p) *n_p2p = ctx->n_p2p;
+  return FMMH_OK;
+}
+
+int fmmh_get_lists(fmmh_ctx* ctx, uint32_t* m2l, uint32_t* p2p) {
+  if (!ctx || !ctx->evaluated) return fail(ctx, FMMH_E_STATE, "no evaluation");
+  return guarded(ctx, [&] {
+    if (m2l && ctx->n_m2l)
+      HCHECK(hipMemcpyAsync(m2l, ctx->m2l.p, (size_t)ctx->n_m2l * 8, hipMemcpyDeviceToHost, ctx->s));
+    if (p2p && ctx->n_p2p)
+      HCHECK(hipMemcpyAsync(p2p, ctx->p2p.p, (size_t)ctx->n_p2p * 8, hipMemcpyDeviceToHost, ctx->s));
+    HCHECK(hipStreamSynchronize(ctx->s));
+  });
+}
+
+static int get_dev(fmmh_ctx* ctx, void* out, const void* src, size_t bytes) {
+  if (!ctx || !ctx->evaluated) return fail(ctx, FMMH_E_STATE, "no evaluation");
+  return guarded(ctx, [&] {
+    HCHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, ctx->s));
+    HCHECK(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int fmmh_get_moments(fmmh_ctx* ctx, double* out) {
+  if (!ctx) return FMMH_E_STATE;
+  return get_dev(ctx, out, ctx->moments.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
+}
+int fmmh_get_locals(fmmh_ctx* ctx, double* out) {
+  if (!ctx) return FMMH_E_STATE;
+  return get_dev(ctx, out, ctx->locals.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
+}
+int fmmh_get_pows(fmmh_ctx* ctx, double* out) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!std::isfinite(ctx->prm.eps_mac)) return fail(ctx, FMMH_E_STATE, "theta-only MAC: no pows");
+  return get_dev(ctx, out, ctx->pows.p, (size_t)ctx->nc * (ctx->prm.p + 1) * 8);
+}
+int fmmh_get_f(fmmh_ctx* ctx, double* f_slot, double* fmin) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!std::isfinite(ctx->prm.eps_mac)) return fail(ctx, FMMH_E_STATE, "theta-only MAC: no f");
+  int rc = f_slot ? get_dev(ctx, f_slot, ctx->fvals.p, ctx->n * 8) : FMMH_OK;
+  if (!rc && fmin) rc = get_dev(ctx, fmin, ctx->fmin.p, (size_t)ctx->nc * 8);
+  return rc;
+}
+
+int fmmh_stage_times(const fmmh_ctx* ctx, double* ms, int max) {
+  if (!ctx) return 0;
+  for (int i = 0; i < ST_COUNT && i < max; i++) ms[i] = ctx->stage_ms[i];
+  return ST_COUNT;
+}
+
+const char* fmmh_stage_name(int stage) {
+  return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
+}
+
+int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8) {
+  if (!ctx) return FMMH_E_STATE;
+  out8[0] = ctx->n;
+  out8[1] = ctx->nc;
+  out8[2] = ctx->nleaves;
+  out8[3] = ctx->n_m2l;
+  out8[4] = ctx->n_p2p;
+  out8[5] = ctx->p2p_interactions;
+  out8[6] = ctx->max_level;
+  out8[7] = ctx->walk_rounds;
+  return FMMH_OK;
+}
+
+int fmmh_direct_sampled(size_t n, const double* pos, const double* mass, double softening, size_t nt,
+                        const uint32_t* idx, double* acc, double* pot, double* f) {
+  if (n == 0 || nt == 0) return FMMH_E_EMPTY;
+  for (size_t t = 0; t < nt; t++)
+    if (idx[t] >= n) return FMMH_E_PARAM;
+  fmmh_params prm;
+  fmmh_default_params(&prm);
+  fmmh_ctx ctx;
+  ctx.prm = prm;
+  if (hipStreamCreateWithFlags(&ctx.s, hipStreamNonBlocking) != hipSuccess) return FMMH_E_HIP;
+  return guarded(&ctx, [&] {
+    ctx.ensure(ctx.in_pos, n * 24);
+    ctx.ensure(ctx.in_mass, n * 8);
+    ctx.ensure(ctx.direct_ids, nt * 4);
+    ctx.ensure(ctx.direct_out, nt * 40);
+    HCHECK(hipMemcpyAsync(ctx.in_pos.p, pos, n * 24, hipMemcpyHostToDevice, ctx.s));
+    HCHECK(hipMemcpyAsync(ctx.in_mass.p, mass, n * 8, hipMemcpyHostToDevice, ctx.s));
+    HCHECK(hipMemcpyAsync(ctx.direct_ids.p, idx, nt * 4, hipMemcpyHostToDevice, ctx.s));
+    launch_direct_targets((const double*)ctx.in_pos.p, (const double*)ctx.in_mass.p, n,
+                          softening * softening, (const uint32_t*)ctx.direct_ids.p, (uint32_t)nt,
+                          (double*)ctx.direct_out.p, ctx.s);
+    std::vector<double> h(nt * 5);
+    HCHECK(hipMemcpyAsync(h.data(), ctx.direct_out.p, nt * 40, hipMemcpyDeviceToHost, ctx.s));
+    HCHECK(hipStreamSynchronize(ctx.s));
+    for (size_t t = 0; t < nt; t++) {
+      acc[3 * t] = h[5 * t];
+      acc[3 * t + 1] = h[5 * t + 1];
+      acc[3 * t + 2] = h[5 * t + 2];
+      if (pot) pot[t] = h[5 * t + 3];
+      if (f) f[t] = h[5 * t + 4];
+    }
+  });
+}
+
+// models.rs:35-58
+void fmmh_plummer(size_t n, const double center[3], uint64_t seed, double* pos, double* mass) {
+  SplitMix rng{seed};
+  const double m = 1.0 / (double)n;
+  const double PI = 3.14159265358979323846;
+  for (size_t i = 0; i < n; i++) {
+    double u = rng.unit();
+    double w = std::pow(u, 2.0 / 3.0);
+    double r = std::sqrt(w / (1.0 - w));
+    double z = 2.0 * rng.unit() - 1.0;
+    double phi = 2.0 * PI * rng.unit();
+    double sn = std::sqrt(1.0 - z * z);
+    double sp, cp;
+    sin_cos(phi, &sp, &cp);
+    pos[3 * i] = center[0] + r * sn * cp;
+    pos[3 * i + 1] = center[1] + r * sn * sp;
+    pos[3 * i + 2] = center[2] + r * z;
+    mass[i] = m;
+  }
+}
+
+// models.rs:61-82
+void fmmh_hernquist(size_t n, const double center[3], uint64_t seed, double* pos, double* mass) {
+  SplitMix rng{seed};
+  const double m = 1.0 / (double)n;
+  const double PI = 3.14159265358979323846;
+  for (size_t i = 0; i < n; i++) {
+    double s = std::sqrt(rng.unit());
+    double r = s / (1.0 - s);
+    double z = 2.0 * rng.unit() - 1.0;
+    double phi = 2.0 * PI * rng.unit();
+    double sp = std::sqrt(1.0 - z * z);
+    double sphi, cphi;
+    sin_cos(phi, &sphi, &cphi);
+    pos[3 * i] = center[0] + r * sp * cphi;
+    pos[3 * i + 1] = center[1] + r * sp * sphi;
+    pos[3 * i + 2] = center[2] + r * z;
+    mass[i] = m;
+  }
+}
+
+}  // extern "C"

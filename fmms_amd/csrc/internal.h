@@ -1,0 +1,144 @@
+// internal.h — host-side declarations shared by the translation units of
+// libfmmh: device-buffer arena, per-stage launchers, the context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fmmh.h"
+#include "common.h"
+
+namespace fmmh {
+
+// ------------------------------------------------------------------ prims
+size_t scan_tmp_words(size_t n);
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp,
+                        hipStream_t s, uint32_t** total_dev);
+size_t radix_tmp_words(size_t n);
+int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* k_alt, uint32_t* v_alt, size_t n,
+                   int bit_lo, int bit_hi, uint32_t* tmp, hipStream_t s);
+
+// Cells in DFS preorder (tree.rs:49-58 as structure of arrays).
+struct DevCells {
+  uint32_t* first;
+  uint32_t* count;
+  int32_t* children;  // [nc][8], -1 absent
+  int32_t* parent;    // -1 for the root
+  uint8_t* leaf;
+  uint8_t* level;
+  double* cx;
+  double* cy;
+  double* cz;
+  double* mass;
+  double* rho;
+  int32_t* rope;        // next cell after this subtree when children run 7..0
+  int32_t* last_child;  // highest-octant child, -1 for leaves
+};
+
+struct DevBodies {  // slot (Morton) order, bodies.rs:13-22
+  double* x;
+  double* y;
+  double* z;
+  double* m;
+};
+
+// BFS (level-order) scratch of the tree build.
+struct DevBfs {
+  uint32_t* first;
+  uint32_t* count;
+  int32_t* parent;  // BFS index of the parent
+  uint8_t* oct;
+  uint8_t* level;
+  uint32_t* dfs;    // DFS id of each BFS cell
+};
+
+// -------------------------------------------------------------- tree.hip
+void launch_bbox(const double* pos, size_t n, double* partial, BBox* out, hipStream_t s);
+void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb,
+                 uint64_t* keys, uint32_t* vals, uint32_t* flags, hipStream_t s);
+void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
+                   DevBodies b, hipStream_t s);
+void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipStream_t s);
+void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
+                        uint32_t n_leaf, uint32_t* nchild, hipStream_t s);
+void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
+                       uint32_t n_leaf, const uint32_t* child_off, uint32_t* chain,
+                       uint8_t* top, hipStream_t s);
+void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8_t* top,
+                    hipStream_t s);
+void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, DevCells c,
+                          uint32_t* lvl_cells, hipStream_t s);
+void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
+                     int center_mode, hipStream_t s);
+void launch_ropes(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, hipStream_t s);
+void launch_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags, hipStream_t s);
+void launch_leaf_compact(DevCells c, uint32_t nc, const uint32_t* off, uint32_t* leaf_ids,
+                         uint32_t* leaf_of_slot, hipStream_t s);
+
+// ------------------------------------------------------------ upward.hip
+bool launch_p2m(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                cplx* moments, hipStream_t s);
+bool launch_m2m(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* moments,
+                hipStream_t s);
+void launch_pows(int p, uint32_t nc, const cplx* moments, const double* fact, double* pows,
+                 hipStream_t s);
+void launch_fprepass(uint32_t n, uint32_t nc, DevCells c, DevBodies b, double* f, hipStream_t s);
+void launch_fmin(const uint32_t* cells, uint32_t cnt, DevCells c, const double* f, double* fmin,
+                 hipStream_t s);
+
+// -------------------------------------------------------------- walk.hip
+struct WalkCounters {
+  uint32_t next;
+  uint32_t m2l;
+  uint32_t p2p;
+  uint32_t overflow;
+};
+struct MacParams {
+  int p;
+  int use_err;
+  double eps;
+  double theta2;
+  const double* pows;
+  const double* binom;
+  const double* fmin;
+};
+void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
+                       uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
+                       DevCells c, MacParams mac, hipStream_t s);
+void launch_directed_keys_m2l(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
+                              hipStream_t s);
+// returns via ctr->next the number of directed P2P entries written
+void launch_directed_keys_p2p(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
+                              uint32_t* count, hipStream_t s);
+void launch_csr_ptr(const uint64_t* keys, uint32_t nnz, int sbits, uint32_t ntarget,
+                    uint32_t* ptr, hipStream_t s);
+
+// --------------------------------------------------------------- m2l.hip
+size_t m2l_max_chunks(uint32_t nnz, uint32_t nc);
+size_t m2l_part_bytes(int p, size_t maxchunks);
+size_t m2l_chunk_bytes();
+void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t* choff,
+                      uint32_t* scan_tmp, void* chunks, hipStream_t s, uint32_t** total_dev);
+bool launch_m2l_chunks(int p, uint32_t nc, const uint32_t* choff, const void* chunks,
+                       uint32_t nchunks, const uint64_t* skeys, int sbits, DevCells c,
+                       const cplx* moments, cplx* part, cplx* locals, hipStream_t s);
+
+// ---------------------------------------------------------- downward.hip
+bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
+                hipStream_t s);
+bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
+                const cplx* locals, double4* far, hipStream_t s);
+
+// -------------------------------------------------------------- near.hip
+void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
+                const double4* far, const uint32_t* order, double* accel, double* pot,
+                hipStream_t s);
+
+// ------------------------------------------------------------ direct.hip
+void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,
+                           const uint32_t* ids, uint32_t nt, double* out /*[nt][5]*/,
+                           hipStream_t s);
+
+}  // namespace fmmh

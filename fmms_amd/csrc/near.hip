@@ -1,0 +1,147 @@
+// near.hip — P2P near field on gfx950 (FP64), fused with the final
+// near + far sum and the slot -> particle-id scatter.
+//
+// Reference: crates/fmm p2p.rs:13-88 (mutual a_i += m_j d/r^3, a_j -= m_i
+// d/r^3, pot += m/r) and translate.rs:339-342 (accel[id] += near + far).
+// Here each target leaf owns its sums: one wave per target leaf walks the
+// leaf's directed source list (its own slots included, self excluded by
+// slot), staging each source leaf's bodies through LDS in tiles of 64. A
+// leaf of c <= 64 bodies uses 64/pow2ceil(c) lanes per target body, each on
+// a strided share of the source tile, combined by xor-shuffles at the end.
+// Softening (an extension; 0 = the reference): r^2 += eps^2.
+#include "common.h"
+#include "internal.h"
+
+namespace fmmh {
+
+__global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_ids,
+                                           const uint32_t* __restrict__ pptr,
+                                           const uint64_t* __restrict__ skeys, uint64_t smask,
+                                           const uint32_t* __restrict__ cfirst,
+                                           const uint32_t* __restrict__ ccount,
+                                           const double* __restrict__ bx,
+                                           const double* __restrict__ by,
+                                           const double* __restrict__ bz,
+                                           const double* __restrict__ bm, double soft2,
+                                           const double4* __restrict__ far,
+                                           const uint32_t* __restrict__ order,
+                                           double* __restrict__ accel, double* __restrict__ pot) {
+  __shared__ double4 tile[64];
+  const int lane = threadIdx.x;
+  const uint32_t leaf = leaf_ids[blockIdx.x];
+  const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
+  const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
+  for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
+    const uint32_t nt = min(64u, nt_all - t0);
+    uint32_t ntp = 1;
+    while (ntp < nt) ntp <<= 1;
+    const uint32_t K = 64 / ntp;
+    const uint32_t ti = lane & (ntp - 1);
+    const uint32_t sub = lane / ntp;
+    const bool tvalid = ti < nt;
+    const uint32_t si = ft + t0 + (tvalid ? ti : 0);
+    const double xi = bx[si], yi = by[si], zi = bz[si];
+    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    for (uint32_t j = jb; j < je; j++) {
+      const uint32_t src = (uint32_t)(skeys[j] & smask);
+      const uint32_t fs = cfirst[src], ns_all = ccount[src];
+      for (uint32_t s0 = 0; s0 < ns_all; s0 += 64) {
+        const uint32_t ns = min(64u, ns_all - s0);
+        __syncthreads();
+        if ((uint32_t)lane < ns) {
+          const uint32_t q = fs + s0 + lane;
+          tile[lane] = make_double4(bx[q], by[q], bz[q], bm[q]);
+        }
+        __syncthreads();
+        const uint32_t sbase = fs + s0;
+        for (uint32_t q = sub; q < ns; q += K) {
+          const double4 sj = tile[q];
+          const double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
+          const bool self = (sbase + q) == si;
+          double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + soft2;
+          r2 = self ? 1.0 : r2;
+          const double mj = self ? 0.0 : sj.w;
+          const double inv = rsqrt(r2);
+          const double minv = mj * inv;
+          const double s3 = minv * inv * inv;
+          ax = fma(s3, dx, ax);
+          ay = fma(s3, dy, ay);
+          az = fma(s3, dz, az);
+          ph += minv;
+        }
+      }
+    }
+    for (uint32_t o = ntp; o < 64; o <<= 1) {
+      ax += __shfl_xor(ax, (int)o, 64);
+      ay += __shfl_xor(ay, (int)o, 64);
+      az += __shfl_xor(az, (int)o, 64);
+      ph += __shfl_xor(ph, (int)o, 64);
+    }
+    if (sub == 0 && tvalid) {
+      const double4 fv = far[si];
+      const uint32_t id = order[si];
+      accel[3 * (size_t)id] = ax + fv.x;
+      accel[3 * (size_t)id + 1] = ay + fv.y;
+      accel[3 * (size_t)id + 2] = az + fv.z;
+      if (pot) pot[id] = ph + fv.w;
+    }
+  }
+}
+
+void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
+                const double4* far, const uint32_t* order, double* accel, double* pot,
+                hipStream_t s) {
+  if (nleaves == 0) return;
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  hipLaunchKernelGGL(k_p2p, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, skeys, smask, c.first,
+                     c.count, b.x, b.y, b.z, b.m, soft2, far, order, accel, pot);
+}
+
+// ------------------------------------------------------------ direct sum
+// FP64 direct sum at sampled targets (direct.rs:6-31 semantics, optional
+// softening): one 256-thread block per target, fixed-order tree reduction.
+__global__ void __launch_bounds__(256) k_direct_targets(const double* __restrict__ pos,
+                                                       const double* __restrict__ mass, size_t n,
+                                                       double soft2,
+                                                       const uint32_t* __restrict__ ids,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[5][256];
+  const uint32_t id = ids[blockIdx.x];
+  const double xi = pos[3 * (size_t)id], yi = pos[3 * (size_t)id + 1], zi = pos[3 * (size_t)id + 2];
+  double ax = 0, ay = 0, az = 0, ph = 0, f = 0;
+  for (size_t j = threadIdx.x; j < n; j += 256) {
+    if (j == id) continue;
+    double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
+    double r2 = dx * dx + dy * dy + dz * dz + soft2;
+    double inv = 1.0 / sqrt(r2);
+    double inv2 = inv * inv;
+    double mj = mass[j];
+    double s3 = mj * inv2 * inv;
+    ax += s3 * dx;
+    ay += s3 * dy;
+    az += s3 * dz;
+    ph += mj * inv;
+    f += mj * inv2;
+  }
+  sh[0][threadIdx.x] = ax;
+  sh[1][threadIdx.x] = ay;
+  sh[2][threadIdx.x] = az;
+  sh[3][threadIdx.x] = ph;
+  sh[4][threadIdx.x] = f;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) out[(size_t)blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,
+                           const uint32_t* ids, uint32_t nt, double* out, hipStream_t s) {
+  if (nt == 0) return;
+  hipLaunchKernelGGL(k_direct_targets, dim3(nt), dim3(256), 0, s, pos, mass, n, soft2, ids, out);
+}
+
+}  // namespace fmmh

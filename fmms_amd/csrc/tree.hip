@@ -1,0 +1,531 @@
+// tree.hip — Morton-key octree build on gfx950 (compiled -ffp-contract=off).
+//
+// Follows crates/fmm tree.rs:94-227 and reproduces its output bit for bit:
+//   bbox (min/max, order-free) -> 63-bit Morton keys with the reference's
+//   quantiser (every axis measured from the x minimum, tree.rs:37-46) ->
+//   stable LSD radix sort (prims.hip) -> level-by-level octant splits ->
+//   DFS-preorder ids -> geometry in the reference's arithmetic order.
+//
+// DFS preorder without recursion: preorder sorts cells by (first slot,
+// level), because the only cells sharing a first slot form a parent->first-
+// child chain at consecutive levels. So id = (cells whose range starts
+// before `first`) + (level - top level of the chain at `first`); the first
+// term is an exclusive scan over slots of the per-slot chain lengths.
+#include "common.h"
+#include "internal.h"
+
+namespace fmmh {
+
+constexpr int BITS = 21;  // tree.rs:16
+constexpr int TB = 256;
+
+__global__ void __launch_bounds__(TB) k_bbox_partial(const double* __restrict__ pos, size_t n,
+                                                    double* __restrict__ partial) {
+  __shared__ double sh[6][TB];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = (size_t)blockIdx.x * TB + threadIdx.x; i < n; i += (size_t)gridDim.x * TB) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double v = pos[3 * i + d];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    sh[d][threadIdx.x] = lo[d];
+    sh[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int st = TB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + st]);
+        sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + st]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// tree.rs:99-111: side = max extent (at least f64::MIN_POSITIVE), inv = 1/side.
+__global__ void k_bbox_final(const double* __restrict__ partial, int nb, BBox* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(lo[d], partial[6 * b + d]);
+      hi[d] = fmax(hi[d], partial[6 * b + 3 + d]);
+    }
+  double side = fmax(fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]), 2.2250738585072014e-308);
+  for (int d = 0; d < 3; d++) {
+    out->lo[d] = lo[d];
+    out->hi[d] = hi[d];
+  }
+  out->side = side;
+  out->inv_side = 1.0 / side;
+}
+
+void launch_bbox(const double* pos, size_t n, double* partial, BBox* out, hipStream_t s) {
+  int nb = (int)((n + TB - 1) / TB);
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(TB), 0, s, pos, n, partial);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, partial, nb, out);
+}
+
+// Spread the low 21 bits of x to every third bit (tree.rs:28-35).
+__device__ __forceinline__ uint64_t dilate21(uint64_t x) {
+  x &= (1ull << BITS) - 1;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  return (x | (x << 2)) & 0x1249249249249249ull;
+}
+
+// tree.rs:37-46: clamp((c - lo[0]) * inv_side, 0, 1) * (2^21-1), truncated;
+// NaN keeps through the clamp and converts to 0.
+__device__ __forceinline__ uint64_t quantise(double c, double lo0, double inv_side) {
+  double v = (c - lo0) * inv_side;
+  if (v < 0.0) v = 0.0;
+  if (v > 1.0) v = 1.0;
+  double w = v * 2097151.0;
+  if (!(w == w)) return 0;
+  return (uint64_t)w;
+}
+
+__global__ void __launch_bounds__(TB) k_keys(const double* __restrict__ pos,
+                                            const double* __restrict__ mass, size_t n,
+                                            const BBox* __restrict__ bb,
+                                            uint64_t* __restrict__ keys,
+                                            uint32_t* __restrict__ vals,
+                                            uint32_t* __restrict__ flags) {
+  size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (i >= n) return;
+  const double lo0 = bb->lo[0], inv = bb->inv_side;
+  double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2], m = mass[i];
+  uint64_t key = dilate21(quantise(z, lo0, inv));
+  key |= dilate21(quantise(y, lo0, inv)) << 1;
+  key |= dilate21(quantise(x, lo0, inv)) << 2;
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+  if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
+}
+
+void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb, uint64_t* keys,
+                 uint32_t* vals, uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, n, bb, keys, vals,
+                     flags);
+}
+
+// Bodies::from_parts (bodies.rs:27-47): id order AoS -> slot order SoA.
+__global__ void __launch_bounds__(TB) k_gather(const double* __restrict__ pos,
+                                              const double* __restrict__ mass,
+                                              const uint32_t* __restrict__ order, size_t n,
+                                              DevBodies b) {
+  size_t k = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (k >= n) return;
+  size_t id = order[k];
+  b.x[k] = pos[3 * id];
+  b.y[k] = pos[3 * id + 1];
+  b.z[k] = pos[3 * id + 2];
+  b.m[k] = mass[id];
+}
+
+void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
+                   DevBodies b, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, order, n, b);
+}
+
+__global__ void k_tree_init(DevBfs bfs, uint32_t n, uint32_t* chain, uint8_t* top) {
+  if (threadIdx.x != 0) return;
+  bfs.first[0] = 0;
+  bfs.count[0] = n;
+  bfs.parent[0] = -1;
+  bfs.oct[0] = 0;
+  bfs.level[0] = 0;
+  chain[0] = 1;
+  top[0] = 0;
+}
+
+void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipStream_t s) {
+  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, bfs, (uint32_t)n, chain, top);
+}
+
+// First index in [lo, hi) whose octant digit is >= oct (digits ascend there).
+__device__ __forceinline__ uint32_t digit_lb(const uint64_t* __restrict__ keys, uint32_t lo,
+                                             uint32_t hi, int shift, uint32_t oct) {
+  while (lo < hi) {
+    uint32_t mid = lo + ((hi - lo) >> 1);
+    if (((keys[mid] >> shift) & 7u) < oct)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Octant boundaries of an internal cell (tree.rs:420-443): bnd[o]..bnd[o+1].
+__device__ __forceinline__ void octant_bounds(const uint64_t* __restrict__ keys, uint32_t f,
+                                              uint32_t c, int level, uint32_t bnd[9]) {
+  const int shift = 3 * (BITS - (level + 1));
+  bnd[0] = f;
+  bnd[8] = f + c;
+#pragma unroll
+  for (uint32_t o = 1; o < 8; o++) bnd[o] = digit_lb(keys, bnd[o - 1], f + c, shift, o);
+}
+
+__device__ __forceinline__ bool is_internal(uint32_t count, int level, uint32_t n_leaf) {
+  return !(count <= n_leaf || level >= BITS);  // tree.rs:414 leaf rule
+}
+
+__global__ void __launch_bounds__(TB) k_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt,
+                                                   const uint64_t* __restrict__ keys,
+                                                   uint32_t n_leaf, uint32_t* __restrict__ nchild) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= cnt) return;
+  uint32_t b = s0 + i;
+  uint32_t f = bfs.first[b], c = bfs.count[b];
+  int lvl = bfs.level[b];
+  uint32_t k = 0;
+  if (is_internal(c, lvl, n_leaf)) {
+    uint32_t bnd[9];
+    octant_bounds(keys, f, c, lvl, bnd);
+#pragma unroll
+    for (int o = 0; o < 8; o++) k += bnd[o + 1] > bnd[o] ? 1u : 0u;
+  }
+  nchild[i] = k;
+}
+
+void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
+                        uint32_t n_leaf, uint32_t* nchild, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_count, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, keys,
+                     n_leaf, nchild);
+}
+
+__global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt,
+                                                  uint32_t e0, const uint64_t* __restrict__ keys,
+                                                  uint32_t n_leaf,
+                                                  const uint32_t* __restrict__ child_off,
+                                                  uint32_t* __restrict__ chain,
+                                                  uint8_t* __restrict__ top) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= cnt) return;
+  uint32_t b = s0 + i;
+  uint32_t f = bfs.first[b], c = bfs.count[b];
+  int lvl = bfs.level[b];
+  if (!is_internal(c, lvl, n_leaf)) return;
+  uint32_t bnd[9];
+  octant_bounds(keys, f, c, lvl, bnd);
+  uint32_t w = e0 + child_off[i];
+  for (int o = 0; o < 8; o++) {
+    uint32_t lo = bnd[o], hi = bnd[o + 1];
+    if (hi <= lo) continue;
+    bfs.first[w] = lo;
+    bfs.count[w] = hi - lo;
+    bfs.parent[w] = (int32_t)b;
+    bfs.oct[w] = (uint8_t)o;
+    bfs.level[w] = (uint8_t)(lvl + 1);
+    // Ranges of one level are disjoint, so each slot is touched at most once
+    // per launch; a slot seen for the first time starts a chain here.
+    if (chain[lo] == 0) top[lo] = (uint8_t)(lvl + 1);
+    chain[lo] += 1;
+    w++;
+  }
+}
+
+void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
+                       uint32_t n_leaf, const uint32_t* child_off, uint32_t* chain, uint8_t* top,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_split_emit, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, e0,
+                     keys, n_leaf, child_off, chain, top);
+}
+
+__global__ void __launch_bounds__(TB) k_dfs_ids(DevBfs bfs, uint32_t nc,
+                                               const uint32_t* __restrict__ cstart,
+                                               const uint8_t* __restrict__ top) {
+  uint32_t b = blockIdx.x * TB + threadIdx.x;
+  if (b >= nc) return;
+  uint32_t f = bfs.first[b];
+  bfs.dfs[b] = cstart[f] + (uint32_t)bfs.level[b] - (uint32_t)top[f];
+}
+
+void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8_t* top,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_dfs_ids, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, bfs, nc, cstart, top);
+}
+
+__global__ void __launch_bounds__(TB) k_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf,
+                                                     DevCells c, uint32_t* __restrict__ lvl_cells) {
+  uint32_t b = blockIdx.x * TB + threadIdx.x;
+  if (b >= nc) return;
+  uint32_t d = bfs.dfs[b];
+  uint32_t cnt = bfs.count[b];
+  int lvl = bfs.level[b];
+  c.first[d] = bfs.first[b];
+  c.count[d] = cnt;
+  c.level[d] = (uint8_t)lvl;
+  c.leaf[d] = is_internal(cnt, lvl, n_leaf) ? 0 : 1;
+  int32_t pb = bfs.parent[b];
+  if (pb >= 0) {
+    uint32_t pd = bfs.dfs[pb];
+    c.parent[d] = (int32_t)pd;
+    c.children[(size_t)pd * 8 + bfs.oct[b]] = (int32_t)d;
+  } else {
+    c.parent[d] = -1;
+  }
+  lvl_cells[b] = d;
+}
+
+void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, DevCells c,
+                          uint32_t* lvl_cells, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_cells, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, bfs, nc, n_leaf, c,
+                     lvl_cells);
+}
+
+// ---------------------------------------------------------------- geometry
+__device__ __forceinline__ double dist3(double ax, double ay, double az, double bx, double by,
+                                        double bz) {  // tree.rs:392-402
+  double dx = ax - bx, dy = ay - by, dz = az - bz;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// Ritter grow/enclose sphere with the bbox-centred fallback, tree.rs:290-389.
+// `get(i, c, r)` yields item i's centre and radius.
+template <class Get>
+__device__ void ritter(int nitems, Get get, double out[3]) {
+  const double MINP = 2.2250738585072014e-308;
+  double c0[3], r0;
+  int a = 0, b = 0;
+  double best = -1.0;
+  double ca[3], ra;
+  get(0, ca, &ra);
+  for (int i = 0; i < nitems; i++) {
+    double ci[3], ri;
+    get(i, ci, &ri);
+    double dx = ci[0] - ca[0], dy = ci[1] - ca[1], dz = ci[2] - ca[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > best) {
+      best = d2;
+      b = i;
+    }
+  }
+  a = b;
+  best = -1.0;
+  get(a, ca, &ra);
+  for (int i = 0; i < nitems; i++) {
+    double ci[3], ri;
+    get(i, ci, &ri);
+    double dx = ci[0] - ca[0], dy = ci[1] - ca[1], dz = ci[2] - ca[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > best) {
+      best = d2;
+      b = i;
+    }
+  }
+  double pa[3], pb[3], rad_a, rad_b;
+  get(a, pa, &rad_a);
+  get(b, pb, &rad_b);
+  double dir[3];
+  {
+    double dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+    double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), MINP);
+    dir[0] = dx / d;
+    dir[1] = dy / d;
+    dir[2] = dz / d;
+  }
+  double fa[3] = {pa[0] + rad_a * dir[0], pa[1] + rad_a * dir[1], pa[2] + rad_a * dir[2]};
+  double fb[3] = {pb[0] - rad_b * dir[0], pb[1] - rad_b * dir[1], pb[2] - rad_b * dir[2]};
+  c0[0] = (fa[0] + fb[0]) * 0.5;
+  c0[1] = (fa[1] + fb[1]) * 0.5;
+  c0[2] = (fa[2] + fb[2]) * 0.5;
+  r0 = dist3(fa[0], fa[1], fa[2], fb[0], fb[1], fb[2]) * 0.5;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int i = 0; i < nitems; i++) {
+      double q[3], rad;
+      get(i, q, &rad);
+      double d = dist3(q[0], q[1], q[2], c0[0], c0[1], c0[2]);
+      if (d + rad > r0) {
+        double dd = fmax(d, MINP);
+        double dr[3] = {(q[0] - c0[0]) / dd, (q[1] - c0[1]) / dd, (q[2] - c0[2]) / dd};
+        double pnt[3] = {q[0] + rad * dr[0], q[1] + rad * dr[1], q[2] + rad * dr[2]};
+        double d_far = d + rad;
+        double t = (d_far - r0) / (2.0 * d_far);
+        for (int k = 0; k < 3; k++) c0[k] += t * (pnt[k] - c0[k]);
+        r0 = (d_far + r0) * 0.5;
+      }
+    }
+  }
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < nitems; i++) {
+    double q[3], rad;
+    get(i, q, &rad);
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(lo[d], q[d] - rad);
+      hi[d] = fmax(hi[d], q[d] + rad);
+    }
+  }
+  double cb[3] = {(lo[0] + hi[0]) * 0.5, (lo[1] + hi[1]) * 0.5, (lo[2] + hi[2]) * 0.5};
+  double rb = 0.0;
+  for (int i = 0; i < nitems; i++) {
+    double q[3], rad;
+    get(i, q, &rad);
+    rb = fmax(rb, dist3(q[0], q[1], q[2], cb[0], cb[1], cb[2]) + rad);
+  }
+  if (rb < r0) {
+    out[0] = cb[0];
+    out[1] = cb[1];
+    out[2] = cb[2];
+  } else {
+    out[0] = c0[0];
+    out[1] = c0[1];
+    out[2] = c0[2];
+  }
+}
+
+// tree.rs:152-227, one thread per cell of a level (children done first).
+__global__ void __launch_bounds__(TB) k_geometry(const uint32_t* __restrict__ cells, uint32_t cnt,
+                                                DevCells c, DevBodies b, int mode) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= cnt) return;
+  uint32_t id = cells[i];
+  double ms = 0.0;
+  double cen[3];
+  double rho = 0.0;
+  if (c.leaf[id]) {
+    uint32_t f = c.first[id], e = f + c.count[id];
+    for (uint32_t k = f; k < e; k++) ms += b.m[k];
+    if (mode == 0) {  // com_of_slots, tree.rs:257-275
+      double com[3] = {0.0, 0.0, 0.0};
+      for (uint32_t k = f; k < e; k++) {
+        double m = b.m[k];
+        com[0] += m * b.x[k];
+        com[1] += m * b.y[k];
+        com[2] += m * b.z[k];
+      }
+      cen[0] = com[0] / ms;
+      cen[1] = com[1] / ms;
+      cen[2] = com[2] / ms;
+    } else {
+      ritter(
+          (int)(e - f),
+          [&](int q, double* qc, double* qr) {
+            qc[0] = b.x[f + q];
+            qc[1] = b.y[f + q];
+            qc[2] = b.z[f + q];
+            *qr = 0.0;
+          },
+          cen);
+    }
+    for (uint32_t k = f; k < e; k++) rho = fmax(rho, dist3(b.x[k], b.y[k], b.z[k], cen[0], cen[1], cen[2]));
+  } else {
+    const int32_t* ch = c.children + (size_t)id * 8;
+    for (int o = 0; o < 8; o++)
+      if (ch[o] >= 0) ms += c.mass[ch[o]];
+    if (mode == 0) {
+      double com[3] = {0.0, 0.0, 0.0};
+      for (int o = 0; o < 8; o++) {
+        int q = ch[o];
+        if (q < 0) continue;
+        double m = c.mass[q];
+        com[0] += m * c.cx[q];
+        com[1] += m * c.cy[q];
+        com[2] += m * c.cz[q];
+      }
+      cen[0] = com[0] / ms;
+      cen[1] = com[1] / ms;
+      cen[2] = com[2] / ms;
+    } else {
+      int list[8];
+      int nl = 0;
+      for (int o = 0; o < 8; o++)
+        if (ch[o] >= 0) list[nl++] = ch[o];
+      ritter(
+          nl,
+          [&](int q, double* qc, double* qr) {
+            int cc = list[q];
+            qc[0] = c.cx[cc];
+            qc[1] = c.cy[cc];
+            qc[2] = c.cz[cc];
+            *qr = c.rho[cc];
+          },
+          cen);
+    }
+    for (int o = 0; o < 8; o++) {
+      int q = ch[o];
+      if (q < 0) continue;
+      rho = fmax(rho, c.rho[q] + dist3(c.cx[q], c.cy[q], c.cz[q], cen[0], cen[1], cen[2]));
+    }
+  }
+  c.mass[id] = ms;
+  c.cx[id] = cen[0];
+  c.cy[id] = cen[1];
+  c.cz[id] = cen[2];
+  c.rho[id] = rho;
+}
+
+void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
+                     int center_mode, hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_geometry, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, lvl_cells, cnt, c, b,
+                     center_mode);
+}
+
+// Ropes of the mirrored DFS (children visited 7..0): the cell visited after
+// a child's subtree is its next lower existing sibling, else the parent's rope.
+__global__ void __launch_bounds__(TB) k_ropes(const uint32_t* __restrict__ cells, uint32_t cnt,
+                                             DevCells c) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= cnt) return;
+  uint32_t id = cells[i];
+  if (id == 0) c.rope[0] = -1;
+  if (c.leaf[id]) {
+    c.last_child[id] = -1;
+    return;
+  }
+  int32_t prev = (id == 0) ? -1 : c.rope[id];
+  const int32_t* ch = c.children + (size_t)id * 8;
+  for (int o = 0; o < 8; o++) {
+    int q = ch[o];
+    if (q < 0) continue;
+    c.rope[q] = prev;
+    prev = q;
+  }
+  c.last_child[id] = prev;
+}
+
+void launch_ropes(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, hipStream_t s) {
+  if (cnt == 0) return;
+  hipLaunchKernelGGL(k_ropes, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, lvl_cells, cnt, c);
+}
+
+__global__ void __launch_bounds__(TB) k_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i < nc) flags[i] = c.leaf[i];
+}
+
+void launch_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_leaf_flags, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, c, nc, flags);
+}
+
+__global__ void __launch_bounds__(TB) k_leaf_compact(DevCells c, uint32_t nc,
+                                                    const uint32_t* __restrict__ off,
+                                                    uint32_t* __restrict__ leaf_ids,
+                                                    uint32_t* __restrict__ leaf_of_slot) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= nc || !c.leaf[i]) return;
+  leaf_ids[off[i]] = i;
+  uint32_t f = c.first[i], e = f + c.count[i];
+  for (uint32_t k = f; k < e; k++) leaf_of_slot[k] = i;
+}
+
+void launch_leaf_compact(DevCells c, uint32_t nc, const uint32_t* off, uint32_t* leaf_ids,
+                         uint32_t* leaf_of_slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_leaf_compact, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, c, nc, off, leaf_ids,
+                     leaf_of_slot);
+}
+
+}  // namespace fmmh

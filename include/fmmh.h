@@ -1,0 +1,135 @@
+/*
+ * fmmh.h — C ABI of the MI355X (gfx950) FMM force evaluator.
+ *
+ * Drop-in for the force evaluation of the reference solver `crates/fmm`
+ * (Vedant9500/FMMS): bodies in (positions, masses, particle-id order),
+ * accelerations and potentials out (particle-id order), at the reference's
+ * expansion order p, MAC tolerance eps, leaf capacity n_leaf and centre mode.
+ *
+ * Conventions (as the reference): G = 1; potential is positive, pot_i =
+ * sum_j m_j / r_ij; acceleration a_i = sum_j m_j (x_j - x_i) / r_ij^3.
+ * All buffers are caller-owned; no allocation crosses the ABI. One context
+ * per host thread; every call is synchronous (returns when results are in
+ * the output buffers), like Fmm::run.
+ *
+ * Reference interfaces replaced (file:line in crates/fmm unless noted):
+ *   fmmh_create/fmmh_build   Fmm::new, Fmm::with_center   fmm.rs:26-49
+ *   fmmh_build (again)       Fmm::refresh                  fmm.rs:54-56
+ *   fmmh_evaluate            Fmm::run / run_threaded       fmm.rs:60-67 -> Result fmm.rs:17-21
+ *   fmmh_accel               Physics::accel (FmmBackend)   crates/visual/src/sim.rs:5-7,124-150
+ *   fmmh_get_tree            Fmm.tree (Octree, Cell)       tree.rs:49-70
+ *   fmmh_get_lists           Recorded {pairs, p2p}         walk.rs:334-340
+ *   fmmh_plummer/hernquist   models::plummer/hernquist     models.rs:35-82
+ *   fmmh_direct_sampled      direct + sampled_error        direct.rs:6-31, crates/bench/src/main.rs:34-78
+ */
+#ifndef FMMH_H
+#define FMMH_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. The reference panics on empty input (tree.rs:95) and on
+ * p > 60 (rotation.rs:126); GpuError::{OverBudget, OverfullLeaf}
+ * (crates/visual/src/gpufmm.rs:1656-1691) are the device-side failures. */
+#define FMMH_OK 0
+#define FMMH_E_EMPTY 1         /* n == 0 */
+#define FMMH_E_PARAM 2         /* p out of range, n_leaf < 1, theta_max <= 0, ... */
+#define FMMH_E_OVER_BUDGET 3   /* device memory would exceed the context budget */
+#define FMMH_E_OVERFULL_LEAF 4 /* reserved: kernels here accept any leaf occupancy */
+#define FMMH_E_NONFINITE 5     /* non-finite position or mass in the input */
+#define FMMH_E_HIP 6           /* a HIP runtime call failed */
+#define FMMH_E_STATE 7         /* evaluate before build, or n differs from the build */
+
+/* Centre modes (tree.rs:19-25) */
+#define FMMH_CENTER_COM 0
+#define FMMH_CENTER_SPHERE 1
+
+/* Precision forms */
+#define FMMH_PREC_FP64 0      /* plain FP64 everywhere (the drop-in) */
+#define FMMH_PREC_FP32BULK 1  /* reserved for the mixed-precision form */
+
+/* Flags */
+#define FMMH_FLAG_TIMING 1u   /* record per-stage device times (hipEvents) */
+
+typedef struct fmmh_params {
+  int p;            /* expansion order (Fmm.p); supported 1..FMMH_MAX_P */
+  double eps_mac;   /* error-controlled MAC tolerance (Fmm.eps, mac.rs:8-11,92);
+                       +INFINITY = theta-only MAC, no f pre-pass */
+  double theta_max; /* opening angle bound; 1.0 = the reference's hard-coded theta<1 */
+  int n_leaf;       /* leaf capacity (Fmm.n_leaf) */
+  double softening; /* Plummer softening of the near field; 0 = the reference (pure 1/r) */
+  int center_mode;  /* FMMH_CENTER_* */
+  int precision;    /* FMMH_PREC_* */
+  unsigned flags;   /* FMMH_FLAG_* */
+  double hbm_budget_bytes; /* device-memory budget of this context; 0 = no limit */
+} fmmh_params;
+
+#define FMMH_MAX_P 12
+
+typedef struct fmmh_ctx fmmh_ctx;
+
+/* Fills the reference's parameters of Fmm::new(parts, p=8, eps=1e-5, n_leaf=16). */
+void fmmh_default_params(fmmh_params* out);
+
+int fmmh_create(const fmmh_params* params, fmmh_ctx** out);
+void fmmh_destroy(fmmh_ctx* ctx);
+const char* fmmh_last_error(const fmmh_ctx* ctx);
+
+/* Host-buffer entry points. pos [n][3] and mass [n] in particle-id order. */
+int fmmh_build(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass);
+int fmmh_evaluate(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass,
+                  double* accel_out /*[n][3]*/, double* pot_out /*[n] or NULL*/);
+int fmmh_accel(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass,
+               double* accel_out, double* pot_out);
+
+/* Device-buffer entry points (pointers into device memory, same layouts).
+ * Results are complete in the output buffers when the call returns. */
+int fmmh_build_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass);
+int fmmh_evaluate_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
+                         double* d_accel, double* d_pot);
+int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
+                      double* d_accel, double* d_pot);
+
+/* Tree of the last build (tree.rs:49-70), DFS-preorder cell ids. Any
+ * output pointer may be NULL. children [nc][8], -1 where absent; cen [nc][3]. */
+size_t fmmh_num_cells(const fmmh_ctx* ctx);
+int fmmh_get_tree(fmmh_ctx* ctx, uint32_t* order, uint64_t* keys_sorted, uint32_t* first,
+                  uint32_t* count, int32_t* children, uint8_t* leaf, uint32_t* level,
+                  double* cen, double* mass, double* rho);
+
+/* Interaction lists of the last evaluation (walk.rs:94-190): mutual M2L
+ * pairs (a,b) and leaf-leaf P2P pairs (a,b) including self pairs (a,a),
+ * each as [k][2] uint32 in an unspecified order. */
+int fmmh_get_list_sizes(fmmh_ctx* ctx, size_t* n_m2l, size_t* n_p2p);
+int fmmh_get_lists(fmmh_ctx* ctx, uint32_t* m2l, uint32_t* p2p);
+
+/* Intermediates of the last evaluation (complex arrays interleaved re,im). */
+int fmmh_get_moments(fmmh_ctx* ctx, double* out /*[nc][(p+1)^2][2]*/);
+int fmmh_get_locals(fmmh_ctx* ctx, double* out /*after L2L*/);
+int fmmh_get_pows(fmmh_ctx* ctx, double* out /*[nc][p+1]*/);
+int fmmh_get_f(fmmh_ctx* ctx, double* f_slot /*[n]*/, double* fmin /*[nc]*/);
+
+/* Per-stage device times (ms) of the last call when FMMH_FLAG_TIMING is set.
+ * Writes up to max entries, returns the number of stages; names via
+ * fmmh_stage_name. */
+int fmmh_stage_times(const fmmh_ctx* ctx, double* ms, int max);
+const char* fmmh_stage_name(int stage);
+/* Work counts of the last evaluation: [n, ncells, nleaves, n_m2l_mutual,
+ * n_p2p_pairs(incl. self), p2p_directed_interactions, max_level, walk_rounds]. */
+int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8);
+
+/* FP64 direct sum at sampled targets, on the device: acc [nt][3], pot [nt]
+ * and f [nt] (= sum m/r^2) for target ids idx[nt]; host arrays. */
+int fmmh_direct_sampled(size_t n, const double* pos, const double* mass, double softening,
+                        size_t nt, const uint32_t* idx, double* acc, double* pot, double* f);
+
+/* Initial conditions (models.rs:13-82, seeded SplitMix64), host arrays. */
+void fmmh_plummer(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
+void fmmh_hernquist(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
