@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""FMM force-evaluation benchmark on one MI355X.
+
+Workload (default): Plummer sphere, N = 1,048,576 equal-mass bodies (total
+mass 1, G = 1, scale radius 1, radii truncated at 20), expansion order p = 8,
+opening angle theta = 0.5 (theta-only MAC), Plummer softening 1e-3, leaf
+capacity autotuned over 16..128, FP64. A step is one full force evaluation
+through the device entry point: tree build (Morton keys, radix sort, cells,
+geometry), upward pass, dual walk, interaction lists, M2L, L2L, L2P, P2P and
+the particle-id scatter, with positions and masses already resident in HBM.
+
+Prints one JSON line: metric, value (body-forces/s), per-step latency
+percentiles, relative force error against the FP64 direct sum on sampled
+targets, the roofline of the dominant kernel (HIP events on the library's
+stream over the timed region) and the CPU baseline (the oracle's threaded
+restatement of Fmm::run_threaded on this host).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-forces/sec @ rel err ≤1e-5, mixed vs FP64 GPU; p50/p99 step latency"
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def t_plus(p):
+    """Complex MACs per directed M2L pair with the m >= 0 half evaluated:
+    sum_n (n+1)(p-n+1)^2."""
+    return sum((n + 1) * (p - n + 1) ** 2 for n in range(p + 1))
+
+
+def t_full(p):
+    """sum_n (2n+1)(p-n+1)^2: complex MACs per directed pair over all (n, m)."""
+    return sum((2 * n + 1) * (p - n + 1) ** 2 for n in range(p + 1))
+
+
+def run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, k):
+    lat = []
+    stages = {}
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        s0 = time.perf_counter()
+        ctx.accel_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
+        lat.append(time.perf_counter() - s0)
+        for name, ms in ctx.stage_times().items():
+            stages.setdefault(name, []).append(ms)
+    hip.synchronize()
+    return time.perf_counter() - t0, lat, stages
+
+
+def autotune_leaf(fa, hip, args, n, d_pos, d_mass, d_acc, d_pot):
+    best = None
+    for nl in [16, 24, 32, 48, 64, 96, 128]:
+        ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=nl, theta_max=args.theta,
+                     softening=args.softening)
+        ctx.accel_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
+        el, _, _ = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, 2)
+        log(f"[autotune] n_leaf={nl}: {1e3 * el / 2:.2f} ms/step")
+        if best is None or el < best[1]:
+            best = (nl, el)
+        ctx.close()
+    return best[0]
+
+
+def cpu_baseline(pos, mass, args, n_leaf):
+    from oracle import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = len(mass)
+    t0 = time.perf_counter()
+    f = O.OracleFmm(pos, mass, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
+                    softening=args.softening)
+    t1 = time.perf_counter()
+    acc, _ = f.run(pos, mass, nthreads=threads)
+    t2 = time.perf_counter()
+    return acc, {
+        "value": n / (t2 - t0),
+        "unit": "body-forces/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one full evaluation of the same workload (N={n}, p={args.p}, theta={args.theta}, "
+                   f"n_leaf={n_leaf}, softening={args.softening}) by the oracle's threaded "
+                   f"restatement of Fmm::run_threaded, tree build included; build {t1 - t0:.2f} s, "
+                   f"evaluation {t2 - t1:.2f} s"),
+        "seconds": t2 - t0,
+        "seconds_excl_build": t2 - t1,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--p", type=int, default=8)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--softening", type=float, default=1e-3)
+    ap.add_argument("--n-leaf", type=int, default=0, help="0 = autotune over 16..128")
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the force evaluation on a single "
+              "MI355X and has no multi-GPU path (see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import fmms_amd as fa
+    from fmms_amd import hip
+
+    n = args.n
+    log(f"[bench] generating Plummer N={n} (r <= 20)")
+    pos, mass = fa.plummer_truncated(n, rmax=20.0, seed=42)
+    d_pos = hip.DeviceArray.from_host(pos)
+    d_mass = hip.DeviceArray.from_host(mass)
+    d_acc = hip.DeviceArray((n, 3))
+    d_pot = hip.DeviceArray((n,))
+
+    n_leaf = args.n_leaf or autotune_leaf(fa, hip, args, n, d_pos, d_mass, d_acc, d_pot)
+    log(f"[bench] n_leaf={n_leaf}")
+    ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
+                 softening=args.softening, timing=True)
+    run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(args.warmup, 1))
+    log(f"[bench] timing {args.steps} steps")
+    elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
+    wc = ctx.work_counts()
+    acc = d_acc.download()
+
+    ms_step = 1e3 * elapsed / args.steps
+    avg = {k: float(np.mean(v)) for k, v in stages.items()}
+    # algorithmic work per launch
+    m2l_dir = 2 * wc["m2l_pairs"]
+    m2l_flops = 8.0 * t_full(args.p) * m2l_dir
+    p2p_flops = 22.0 * wc["p2p_interactions"]
+    kernels = {
+        "m2l": ("valu", m2l_flops, "TFLOP/s"),
+        "p2p": ("valu", p2p_flops, "TFLOP/s"),
+    }
+    stage_roof = {}
+    for k, (bound, work, unit) in kernels.items():
+        if avg.get(k, 0) > 0:
+            ach = work / (avg[k] * 1e-3) / 1e12
+            stage_roof[k] = {"ms": avg[k], "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                             "frac": ach / FP64_PEAK_TFLOPS}
+    dom = max(stage_roof, key=lambda k: stage_roof[k]["ms"])
+    roof = {"kernel": dom, "bound": kernels[dom][0], "achieved": round(stage_roof[dom]["achieved"], 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(stage_roof[dom]["frac"], 4), "traffic": None,
+            "work_per_launch": kernels[dom][1], "avg_launch_ms": round(stage_roof[dom]["ms"], 4)}
+
+    err = None
+    if not args.no_error:
+        ids = fa.sample_ids(n, args.samples, 7)
+        log(f"[bench] FP64 direct sum on {len(ids)} sampled targets")
+        a_ref, _, f_ref = fa.direct_sampled(pos, mass, ids, softening=args.softening)
+        rel = fa.rel_errors(acc[ids], a_ref)
+        scaled = np.linalg.norm(acc[ids] - a_ref, axis=1) / f_ref
+        err = {"targets": int(len(ids)), "rel_median": float(np.median(rel)),
+               "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max()),
+               "scaled_rms": float(np.sqrt(np.mean(scaled ** 2))), "scaled_max": float(scaled.max())}
+        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle, threaded)")
+        acc_cpu, cpu = cpu_baseline(pos, mass, args, n_leaf)
+        d = np.linalg.norm(acc - acc_cpu, axis=1) / np.linalg.norm(acc_cpu, axis=1)
+        cpu["max_rel_diff_vs_gpu"] = float(d.max())
+
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / elapsed,
+        "unit": "body-forces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Plummer sampler, radii truncated at 20)",
+        "config": {"workload": f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_fp64",
+                   "n": n, "p": args.p, "theta": args.theta, "eps_mac": "inf",
+                   "softening": args.softening, "n_leaf": n_leaf, "precision": "fp64",
+                   "form": "plain FP64 GPU drop-in"},
+        "error": err,
+        "roofline": roof,
+        "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+        "stage_roofline": stage_roof,
+        "work": wc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,7 @@ Mirrors the reference solver API `crates/fmm` (Vedant9500/FMMS):
 A particle list `&[Particle]` becomes two arrays: pos [n, 3] and mass [n]
 (float64, particle-id order). Everything runs on the GPU through the C ABI
 declared in include/fmmh.h; there is no CPU fallback. If libfmmh.so is
-missing or cannot be loaded, importing this package raises.
+missing or cannot be loaded, the first call into it raises.
 """
 import ctypes as ct
 import math
@@ -92,6 +92,7 @@ SIGNATURES = {
     "fmmh_direct_sampled": (ct.c_int, [_sz, _pd, _pd, ct.c_double, _sz, _pu32, _pd, _pd, _pd]),
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
+    "fmmh_plummer_trunc": (None, [_sz, _pd, ct.c_double, ct.c_uint64, _pd, _pd]),
 }
 
 
@@ -106,10 +107,14 @@ def _load():
     return lib
 
 
-_lib = _load()
+_lib = None
 
 
 def lib():
+    """The loaded libfmmh.so (raises if it is missing or stale)."""
+    global _lib
+    if _lib is None:
+        _lib = _load()
     return _lib
 
 
@@ -144,7 +149,7 @@ class Fmm:
     def __init__(self, pos, mass, p, eps, n_leaf, center_mode=CenterMode.Com, theta_max=1.0,
                  softening=0.0, timing=False, hbm_budget_bytes=0.0):
         prm = _Params()
-        _lib.fmmh_default_params(ct.byref(prm))
+        lib().fmmh_default_params(ct.byref(prm))
         prm.p, prm.eps_mac, prm.theta_max, prm.n_leaf = int(p), float(eps), float(theta_max), int(n_leaf)
         prm.softening, prm.center_mode = float(softening), int(center_mode)
         prm.flags = FMMH_FLAG_TIMING if timing else 0
@@ -152,7 +157,7 @@ class Fmm:
         self.p, self.eps, self.n_leaf = int(p), float(eps), int(n_leaf)
         self.theta_max, self.softening, self.center_mode = float(theta_max), float(softening), int(center_mode)
         h = _vp()
-        rc = _lib.fmmh_create(ct.byref(prm), ct.byref(h))
+        rc = lib().fmmh_create(ct.byref(prm), ct.byref(h))
         if rc != FMMH_OK:
             raise FmmError(rc, "invalid parameters")
         self._h = h
@@ -166,7 +171,7 @@ class Fmm:
 
     def close(self):
         if getattr(self, "_h", None):
-            _lib.fmmh_destroy(self._h)
+            lib().fmmh_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -174,18 +179,18 @@ class Fmm:
 
     def _check(self, rc):
         if rc != FMMH_OK:
-            raise FmmError(rc, _lib.fmmh_last_error(self._h).decode())
+            raise FmmError(rc, lib().fmmh_last_error(self._h).decode())
 
     def refresh(self, pos, mass):
         mass = _f64(mass)
         n = mass.shape[0]
         pos = _f64(pos, (n, 3)) if n else _f64(pos)
-        self._check(_lib.fmmh_build(self._h, n, _p(pos), _p(mass)))
+        self._check(lib().fmmh_build(self._h, n, _p(pos), _p(mass)))
         self.n = n
 
     @property
     def n_cells(self):
-        return int(_lib.fmmh_num_cells(self._h))
+        return int(lib().fmmh_num_cells(self._h))
 
     def run(self, pos, mass):
         mass = _f64(mass)
@@ -193,7 +198,7 @@ class Fmm:
         pos = _f64(pos, (n, 3))
         acc = np.empty((n, 3))
         pot = np.empty(n)
-        self._check(_lib.fmmh_evaluate(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
+        self._check(lib().fmmh_evaluate(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
         return Result(acc, pot, self.n_cells)
 
     def run_threaded(self, pos, mass, nthreads=1):
@@ -207,22 +212,22 @@ class Fmm:
         pos = _f64(pos, (n, 3))
         acc = np.empty((n, 3))
         pot = np.empty(n)
-        self._check(_lib.fmmh_accel(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
+        self._check(lib().fmmh_accel(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
         self.n = n
         return Result(acc, pot, self.n_cells)
 
     # ---- device-resident entry points (raw device pointers, e.g. torch .data_ptr())
     def accel_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr=0):
-        self._check(_lib.fmmh_accel_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
+        self._check(lib().fmmh_accel_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
                                            _vp(pot_ptr) if pot_ptr else None))
         self.n = n
 
     def build_device(self, n, pos_ptr, mass_ptr):
-        self._check(_lib.fmmh_build_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr)))
+        self._check(lib().fmmh_build_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr)))
         self.n = n
 
     def evaluate_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr=0):
-        self._check(_lib.fmmh_evaluate_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
+        self._check(lib().fmmh_evaluate_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
                                               _vp(pot_ptr) if pot_ptr else None))
 
     # ---- parity views
@@ -231,7 +236,7 @@ class Fmm:
         t = Tree(np.empty(n, np.uint32), np.empty(n, np.uint64), np.empty(nc, np.uint32),
                  np.empty(nc, np.uint32), np.empty((nc, 8), np.int32), np.empty(nc, np.uint8),
                  np.empty(nc, np.uint32), np.empty((nc, 3)), np.empty(nc), np.empty(nc))
-        self._check(_lib.fmmh_get_tree(self._h, _p(t.order, _pu32), _p(t.keys, _pu64),
+        self._check(lib().fmmh_get_tree(self._h, _p(t.order, _pu32), _p(t.keys, _pu64),
                                        _p(t.first, _pu32), _p(t.count, _pu32),
                                        _p(t.children, _pi32), _p(t.leaf, _pu8),
                                        _p(t.level, _pu32), _p(t.cen), _p(t.mass), _p(t.rho)))
@@ -239,41 +244,41 @@ class Fmm:
 
     def lists(self):
         a, b = _sz(), _sz()
-        self._check(_lib.fmmh_get_list_sizes(self._h, ct.byref(a), ct.byref(b)))
+        self._check(lib().fmmh_get_list_sizes(self._h, ct.byref(a), ct.byref(b)))
         m2l = np.empty((a.value, 2), np.uint32)
         p2p = np.empty((b.value, 2), np.uint32)
-        self._check(_lib.fmmh_get_lists(self._h, _p(m2l, _pu32), _p(p2p, _pu32)))
+        self._check(lib().fmmh_get_lists(self._h, _p(m2l, _pu32), _p(p2p, _pu32)))
         return m2l, p2p
 
     def moments(self):
         out = np.empty((self.n_cells, dim(self.p)), np.complex128)
-        self._check(_lib.fmmh_get_moments(self._h, out.ctypes.data_as(_pd)))
+        self._check(lib().fmmh_get_moments(self._h, out.ctypes.data_as(_pd)))
         return out
 
     def locals(self):
         out = np.empty((self.n_cells, dim(self.p)), np.complex128)
-        self._check(_lib.fmmh_get_locals(self._h, out.ctypes.data_as(_pd)))
+        self._check(lib().fmmh_get_locals(self._h, out.ctypes.data_as(_pd)))
         return out
 
     def pows(self):
         out = np.empty((self.n_cells, self.p + 1))
-        self._check(_lib.fmmh_get_pows(self._h, _p(out)))
+        self._check(lib().fmmh_get_pows(self._h, _p(out)))
         return out
 
     def f(self):
         f = np.empty(self.n)
         fmin = np.empty(self.n_cells)
-        self._check(_lib.fmmh_get_f(self._h, _p(f), _p(fmin)))
+        self._check(lib().fmmh_get_f(self._h, _p(f), _p(fmin)))
         return f, fmin
 
     def stage_times(self):
         buf = (ct.c_double * 32)()
-        k = _lib.fmmh_stage_times(self._h, buf, 32)
-        return {_lib.fmmh_stage_name(i).decode(): buf[i] for i in range(k)}
+        k = lib().fmmh_stage_times(self._h, buf, 32)
+        return {lib().fmmh_stage_name(i).decode(): buf[i] for i in range(k)}
 
     def work_counts(self):
         out = np.zeros(8, np.uint64)
-        self._check(_lib.fmmh_work_counts(self._h, _p(out, _pu64)))
+        self._check(lib().fmmh_work_counts(self._h, _p(out, _pu64)))
         keys = ["n", "ncells", "nleaves", "m2l_pairs", "p2p_pairs", "p2p_interactions",
                 "max_level", "walk_rounds"]
         return dict(zip(keys, (int(v) for v in out)))
@@ -284,7 +289,16 @@ def plummer(n, center=(0.0, 0.0, 0.0), seed=42):
     pos = np.empty((n, 3))
     mass = np.empty(n)
     c = _f64(center)
-    _lib.fmmh_plummer(n, _p(c), seed, _p(pos), _p(mass))
+    lib().fmmh_plummer(n, _p(c), seed, _p(pos), _p(mass))
+    return pos, mass
+
+
+def plummer_truncated(n, rmax=20.0, center=(0.0, 0.0, 0.0), seed=42):
+    """Plummer sphere with radii redrawn until r <= rmax (total mass 1)."""
+    pos = np.empty((n, 3))
+    mass = np.empty(n)
+    c = _f64(center)
+    lib().fmmh_plummer_trunc(n, _p(c), float(rmax), seed, _p(pos), _p(mass))
     return pos, mass
 
 
@@ -292,7 +306,7 @@ def hernquist(n, center=(0.0, 0.0, 0.0), seed=7):
     pos = np.empty((n, 3))
     mass = np.empty(n)
     c = _f64(center)
-    _lib.fmmh_hernquist(n, _p(c), seed, _p(pos), _p(mass))
+    lib().fmmh_hernquist(n, _p(c), seed, _p(pos), _p(mass))
     return pos, mass
 
 
@@ -317,7 +331,7 @@ def direct_sampled(pos, mass, ids, softening=0.0):
     acc = np.empty((k, 3))
     pot = np.empty(k)
     f = np.empty(k)
-    rc = _lib.fmmh_direct_sampled(n, _p(pos), _p(mass), float(softening), k, _p(ids, _pu32),
+    rc = lib().fmmh_direct_sampled(n, _p(pos), _p(mass), float(softening), k, _p(ids, _pu32),
                                   _p(acc), _p(pot), _p(f))
     if rc != FMMH_OK:
         raise FmmError(rc, "direct_sampled failed")

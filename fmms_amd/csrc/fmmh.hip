@@ -29,13 +29,14 @@ enum Stage {
   ST_WALK,
   ST_LISTS,
   ST_M2L,
+  ST_M2L_REDUCE,
   ST_L2L,
   ST_L2P,
   ST_P2P,
   ST_COUNT
 };
-const char* kStageNames[ST_COUNT] = {"tree",  "upward", "prepass", "walk", "lists",
-                                     "m2l",   "l2l",    "l2p",     "p2p"};
+const char* kStageNames[ST_COUNT] = {"tree", "upward",     "prepass", "walk", "lists",
+                                     "m2l",  "m2l_reduce", "l2l",     "l2p",  "p2p"};
 
 struct Buf {
   void* p = nullptr;
@@ -94,7 +95,7 @@ struct fmmh_ctx {
   Buf front_a, front_b, m2l, p2p, ctr;
   Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
   Buf nch, choff, chunks, part;
-  Buf far;
+  Buf far, p2p_work;
   Buf direct_ids, direct_out;
   Buf pinned;
 
@@ -107,7 +108,7 @@ struct fmmh_ctx {
                   &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &direct_ids, &direct_out};
+                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &direct_ids, &direct_out};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -392,9 +393,14 @@ struct fmmh_ctx {
     HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
     uint32_t nchunks = read1(tot);
     ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
+    ensure(p2p_work, 64);
+    HCHECK(hipMemsetAsync(p2p_work.p, 0, 8, s));
+    launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
+                     (unsigned long long*)p2p_work.p, s);
     mark(ST_M2L);
-    launch_m2l_chunks(p, nc, (uint32_t*)choff.p, chunks.p, nchunks, msorted, sbits, c, mom,
-                      (cplx*)part.p, loc, s);
+    launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, (cplx*)part.p, s);
+    mark(ST_M2L_REDUCE);
+    launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
     mark(ST_L2L);
     for (int L = 1; L <= (int)max_level; L++)
       launch_l2l(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, loc, s);
@@ -405,7 +411,7 @@ struct fmmh_ctx {
     launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, sbits, c, b,
                prm.softening * prm.softening, (double4*)far.p, order, acc, pot, s);
     mark(ST_COUNT);
-    HCHECK(hipStreamSynchronize(s));
+    p2p_interactions = read1((const unsigned long long*)p2p_work.p);
     HCHECK(hipGetLastError());
     evaluated = true;
   }
@@ -815,6 +821,32 @@ void fmmh_plummer(size_t n, const double center[3], uint64_t seed, double* pos, 
     double u = rng.unit();
     double w = std::pow(u, 2.0 / 3.0);
     double r = std::sqrt(w / (1.0 - w));
+    double z = 2.0 * rng.unit() - 1.0;
+    double phi = 2.0 * PI * rng.unit();
+    double sn = std::sqrt(1.0 - z * z);
+    double sp, cp;
+    sin_cos(phi, &sp, &cp);
+    pos[3 * i] = center[0] + r * sn * cp;
+    pos[3 * i + 1] = center[1] + r * sn * sp;
+    pos[3 * i + 2] = center[2] + r * z;
+    mass[i] = m;
+  }
+}
+
+// Plummer sphere truncated at r <= rmax: models.rs:35-58 with the radius
+// redrawn (fresh u) until it falls inside rmax; total mass 1, equal masses.
+void fmmh_plummer_trunc(size_t n, const double center[3], double rmax, uint64_t seed, double* pos,
+                        double* mass) {
+  SplitMix rng{seed};
+  const double m = 1.0 / (double)n;
+  const double PI = 3.14159265358979323846;
+  for (size_t i = 0; i < n; i++) {
+    double r;
+    do {
+      double u = rng.unit();
+      double w = std::pow(u, 2.0 / 3.0);
+      r = std::sqrt(w / (1.0 - w));
+    } while (!(r <= rmax));
     double z = 2.0 * rng.unit() - 1.0;
     double phi = 2.0 * PI * rng.unit();
     double sn = std::sqrt(1.0 - z * z);

@@ -120,9 +120,10 @@ size_t m2l_part_bytes(int p, size_t maxchunks);
 size_t m2l_chunk_bytes();
 void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t* choff,
                       uint32_t* scan_tmp, void* chunks, hipStream_t s, uint32_t** total_dev);
-bool launch_m2l_chunks(int p, uint32_t nc, const uint32_t* choff, const void* chunks,
-                       uint32_t nchunks, const uint64_t* skeys, int sbits, DevCells c,
-                       const cplx* moments, cplx* part, cplx* locals, hipStream_t s);
+bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
+                       int sbits, DevCells c, const cplx* moments, cplx* part, hipStream_t s);
+bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
+                       hipStream_t s);
 
 // ---------------------------------------------------------- downward.hip
 bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
@@ -135,6 +136,9 @@ void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr
                 const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
                 const double4* far, const uint32_t* order, double* accel, double* pot,
                 hipStream_t s);
+// Directed body-body interactions of a P2P pair list (self pairs c(c-1)).
+void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
+                      unsigned long long* out, hipStream_t s);
 
 // ------------------------------------------------------------ direct.hip
 void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,

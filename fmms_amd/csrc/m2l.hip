@@ -193,12 +193,16 @@ size_t m2l_max_chunks(uint32_t nnz, uint32_t nc) { return (size_t)nnz / M2L_CH +
 size_t m2l_part_bytes(int p, size_t maxchunks) { return maxchunks * hnpos(p) * sizeof(cplx); }
 
 template <int P>
-static void m2l_t(uint32_t nc, const uint32_t* choff, const M2lChunk* chunks, uint32_t nchunks,
-                  const uint64_t* skeys, uint64_t smask, DevCells c, const cplx* moments,
-                  cplx* part, cplx* locals, hipStream_t s) {
+static void m2l_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
+                  DevCells c, const cplx* moments, cplx* part, hipStream_t s) {
   if (nchunks)
     hipLaunchKernelGGL(k_m2l<P>, dim3((nchunks + 63) / 64), dim3(64), 0, s, chunks, nchunks, skeys,
                        smask, c.cx, c.cy, c.cz, moments, part);
+}
+
+template <int P>
+static void m2l_reduce_t(uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
+                         hipStream_t s) {
   size_t th = (size_t)nc * hnpos(P);
   hipLaunchKernelGGL(k_m2l_reduce<P>, dim3((th + 255) / 256), dim3(256), 0, s, nc, choff, part,
                      locals);
@@ -212,27 +216,37 @@ void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t
                      (M2lChunk*)chunks);
 }
 
-bool launch_m2l_chunks(int p, uint32_t nc, const uint32_t* choff, const void* chunks,
-                       uint32_t nchunks, const uint64_t* skeys, int sbits, DevCells c,
-                       const cplx* moments, cplx* part, cplx* locals, hipStream_t s) {
+#define FMMH_P_SWITCH3(p, CALL) \
+  switch (p) {                  \
+    case 1: CALL(1); break;     \
+    case 2: CALL(2); break;     \
+    case 3: CALL(3); break;     \
+    case 4: CALL(4); break;     \
+    case 5: CALL(5); break;     \
+    case 6: CALL(6); break;     \
+    case 7: CALL(7); break;     \
+    case 8: CALL(8); break;     \
+    case 9: CALL(9); break;     \
+    case 10: CALL(10); break;   \
+    case 11: CALL(11); break;   \
+    case 12: CALL(12); break;   \
+    default: return false;      \
+  }
+
+bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
+                       int sbits, DevCells c, const cplx* moments, cplx* part, hipStream_t s) {
   const uint64_t smask = (1ull << sbits) - 1ull;
   const M2lChunk* ck = (const M2lChunk*)chunks;
-#define CALL(P) m2l_t<P>(nc, choff, ck, nchunks, skeys, smask, c, moments, part, locals, s)
-  switch (p) {
-    case 1: CALL(1); break;
-    case 2: CALL(2); break;
-    case 3: CALL(3); break;
-    case 4: CALL(4); break;
-    case 5: CALL(5); break;
-    case 6: CALL(6); break;
-    case 7: CALL(7); break;
-    case 8: CALL(8); break;
-    case 9: CALL(9); break;
-    case 10: CALL(10); break;
-    case 11: CALL(11); break;
-    case 12: CALL(12); break;
-    default: return false;
-  }
+#define CALL(P) m2l_t<P>(ck, nchunks, skeys, smask, c, moments, part, s)
+  FMMH_P_SWITCH3(p, CALL)
+#undef CALL
+  return true;
+}
+
+bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
+                       hipStream_t s) {
+#define CALL(P) m2l_reduce_t<P>(nc, choff, part, locals, s)
+  FMMH_P_SWITCH3(p, CALL)
 #undef CALL
   return true;
 }

@@ -98,6 +98,26 @@ void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr
                      c.count, b.x, b.y, b.z, b.m, soft2, far, order, accel, pot);
 }
 
+__global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,
+                                                  const uint32_t* __restrict__ count,
+                                                  unsigned long long* __restrict__ out) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < np) {
+    uint2 pr = pairs[i];
+    unsigned long long ca = count[pr.x], cb = count[pr.y];
+    v = pr.x == pr.y ? ca * (ca - 1) : 2 * ca * cb;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
+                      unsigned long long* out, hipStream_t s) {
+  if (np == 0) return;
+  hipLaunchKernelGGL(k_p2p_count, dim3((np + 255) / 256), dim3(256), 0, s, pairs, np, count, out);
+}
+
 // ------------------------------------------------------------ direct sum
 // FP64 direct sum at sampled targets (direct.rs:6-31 semantics, optional
 // softening): one 256-thread block per target, fixed-order tree reduction.

@@ -128,6 +128,10 @@ int fmmh_direct_sampled(size_t n, const double* pos, const double* mass, double 
 /* Initial conditions (models.rs:13-82, seeded SplitMix64), host arrays. */
 void fmmh_plummer(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
 void fmmh_hernquist(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
+/* Plummer truncated at r <= rmax (radius redrawn until inside); an extension
+ * of models.rs:35-58 for the r <= 20 workloads. */
+void fmmh_plummer_trunc(size_t n, const double center[3], double rmax, uint64_t seed, double* pos,
+                        double* mass);
 
 #ifdef __cplusplus
 }
