@@ -95,7 +95,7 @@ struct fmmh_ctx {
   Buf front_a, front_b, m2l, p2p, ctr;
   Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
   Buf nch, choff, chunks, part;
-  Buf far, p2p_work;
+  Buf far, p2p_work, walkrec;
   Buf direct_ids, direct_out;
   Buf pinned;
 
@@ -108,7 +108,7 @@ struct fmmh_ctx {
                   &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &direct_ids, &direct_out};
+                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &direct_ids, &direct_out};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -298,6 +298,8 @@ struct fmmh_ctx {
                       bodies(), prm.center_mode, s);
     for (int L = 0; L <= (int)max_level; L++)
       launch_ropes((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(), s);
+    ensure(walkrec, (size_t)nc * walk_rec_bytes());
+    launch_pack_walk(nc, cells(), walkrec.p, s);
     launch_leaf_flags(cells(), nc, (uint32_t*)leaf_flags.p, s);
     uint32_t* tot = nullptr;
     scan_exclusive_u32((uint32_t*)leaf_flags.p, (uint32_t*)leaf_off.p, nc, (uint32_t*)scan_tmp.p, s,
@@ -393,12 +395,12 @@ struct fmmh_ctx {
     HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
     uint32_t nchunks = read1(tot);
     ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
-    ensure(p2p_work, 64);
-    HCHECK(hipMemsetAsync(p2p_work.p, 0, 8, s));
+    ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
     launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
                      (unsigned long long*)p2p_work.p, s);
     mark(ST_M2L);
-    launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, (cplx*)part.p, s);
+    launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, prm.softening * prm.softening,
+                      (cplx*)part.p, s);
     mark(ST_M2L_REDUCE);
     launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
     mark(ST_L2L);
@@ -447,7 +449,8 @@ struct fmmh_ctx {
       HCHECK(hipMemcpyAsync(ctr.p, hc, sizeof init, hipMemcpyHostToDevice, s));
       launch_walk_round((const uint2*)fin->p, nf, (uint2*)fout->p, (uint32_t)(fout->cap / 8),
                         (uint2*)m2l.p, (uint32_t)(m2l.cap / 8), (uint2*)p2p.p,
-                        (uint32_t)(p2p.cap / 8), (WalkCounters*)ctr.p, cells(), mp, s);
+                        (uint32_t)(p2p.cap / 8), (WalkCounters*)ctr.p, walkrec.p,
+                        (const double*)c_mass.p, mp, s);
       HCHECK(hipMemcpyAsync(hc, ctr.p, sizeof init, hipMemcpyDeviceToHost, s));
       HCHECK(hipStreamSynchronize(s));
       WalkCounters got = *hc;

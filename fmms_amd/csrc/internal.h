@@ -103,9 +103,11 @@ struct MacParams {
   const double* binom;
   const double* fmin;
 };
+size_t walk_rec_bytes();
+void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s);
 void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
                        uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
-                       DevCells c, MacParams mac, hipStream_t s);
+                       const void* rec, const double* mass, MacParams mac, hipStream_t s);
 void launch_directed_keys_m2l(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
                               hipStream_t s);
 // returns via ctr->next the number of directed P2P entries written
@@ -121,7 +123,8 @@ size_t m2l_chunk_bytes();
 void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t* choff,
                       uint32_t* scan_tmp, void* chunks, hipStream_t s, uint32_t** total_dev);
 bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
-                       int sbits, DevCells c, const cplx* moments, cplx* part, hipStream_t s);
+                       int sbits, DevCells c, const cplx* moments, double soft2, cplx* part,
+                       hipStream_t s);
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s);
 
@@ -136,7 +139,9 @@ void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr
                 const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
                 const double4* far, const uint32_t* order, double* accel, double* pot,
                 hipStream_t s);
-// Directed body-body interactions of a P2P pair list (self pairs c(c-1)).
+// Directed body-body interactions of a P2P pair list (self pairs c(c-1));
+// out[0] = total, out needs 1 + p2p_count_scratch(np) words.
+size_t p2p_count_scratch(uint32_t np);
 void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
                       unsigned long long* out, hipStream_t s);
 

@@ -29,9 +29,11 @@ __device__ __forceinline__ void cmac(cplx& f, double ar, double ai, double br, d
 
 // F (m>=0 half, hpos layout) += M2L of source multipole M (m>=0 half) at r.
 template <int P>
-__device__ __forceinline__ void m2l_accumulate(double x, double y, double z, const cplx* M,
-                                               cplx* F) {
-  const double r2 = x * x + y * y + z * z;
+__device__ __forceinline__ void m2l_accumulate(double x, double y, double z, double soft2,
+                                               const cplx* M, cplx* F) {
+  // Softening: the recursion with r^2 -> r^2 + eps^2 gives the traceless
+  // derivatives of the Plummer kernel (soft2 = 0: pure 1/r).
+  const double r2 = x * x + y * y + z * z + soft2;
   const double ir = rsqrt(r2);
   const double ir2 = ir * ir;
   cplx T[P + 1], T1[P + 1], T2[P + 1];
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(64) k_m2l(const M2lChunk* __restrict__ chunks,
                                            const double* __restrict__ ccx,
                                            const double* __restrict__ ccy,
                                            const double* __restrict__ ccz,
-                                           const cplx* __restrict__ moments,
+                                           const cplx* __restrict__ moments, double soft2,
                                            cplx* __restrict__ part) {
   constexpr int D = hdim(P), NF = hnpos(P);
   const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(64) k_m2l(const M2lChunk* __restrict__ chunks,
     for (int k = 0; k <= P; k++)
 #pragma unroll
       for (int l = 0; l <= k; l++) M[hpos(k, l)] = ms[hidx(k, l)];
-    m2l_accumulate<P>(tx - ccx[src], ty - ccy[src], tz - ccz[src], M, F);
+    m2l_accumulate<P>(tx - ccx[src], ty - ccy[src], tz - ccz[src], soft2, M, F);
   }
   cplx* out = part + (size_t)ci * NF;
 #pragma unroll
@@ -194,10 +196,10 @@ size_t m2l_part_bytes(int p, size_t maxchunks) { return maxchunks * hnpos(p) * s
 
 template <int P>
 static void m2l_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
-                  DevCells c, const cplx* moments, cplx* part, hipStream_t s) {
+                  DevCells c, const cplx* moments, double soft2, cplx* part, hipStream_t s) {
   if (nchunks)
     hipLaunchKernelGGL(k_m2l<P>, dim3((nchunks + 63) / 64), dim3(64), 0, s, chunks, nchunks, skeys,
-                       smask, c.cx, c.cy, c.cz, moments, part);
+                       smask, c.cx, c.cy, c.cz, moments, soft2, part);
 }
 
 template <int P>
@@ -234,10 +236,11 @@ void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t
   }
 
 bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
-                       int sbits, DevCells c, const cplx* moments, cplx* part, hipStream_t s) {
+                       int sbits, DevCells c, const cplx* moments, double soft2, cplx* part,
+                       hipStream_t s) {
   const uint64_t smask = (1ull << sbits) - 1ull;
   const M2lChunk* ck = (const M2lChunk*)chunks;
-#define CALL(P) m2l_t<P>(ck, nchunks, skeys, smask, c, moments, part, s)
+#define CALL(P) m2l_t<P>(ck, nchunks, skeys, smask, c, moments, soft2, part, s)
   FMMH_P_SWITCH3(p, CALL)
 #undef CALL
   return true;

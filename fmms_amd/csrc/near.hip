@@ -14,6 +14,35 @@
 
 namespace fmmh {
 
+// Bodies of a 64-slot LDS tile against the lane's target: lane (ti, sub)
+// takes tile entries sub, sub+K, ... (K = 64 / pow2ceil(targets)). Entries
+// past the sources are padded with mass 0 at 1e100, which contributes 0.
+template <bool SELF>
+__device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint32_t sub,
+                                         uint32_t K, double xi, double yi, double zi, double soft2,
+                                         int self_j, double& ax, double& ay, double& az,
+                                         double& ph) {
+#pragma unroll 4
+  for (uint32_t j = sub; j < 64; j += K) {
+    const double4 sj = tile[j];
+    const double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
+    double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + soft2;
+    double mj = sj.w;
+    if (SELF) {
+      const bool self = (int)j == self_j;
+      r2 = self ? 1.0 : r2;
+      mj = self ? 0.0 : mj;
+    }
+    const double inv = rsqrt(r2);
+    const double minv = mj * inv;
+    const double s3 = minv * inv * inv;
+    ax = fma(s3, dx, ax);
+    ay = fma(s3, dy, ay);
+    az = fma(s3, dz, az);
+    ph += minv;
+  }
+}
+
 __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_ids,
                                            const uint32_t* __restrict__ pptr,
                                            const uint64_t* __restrict__ skeys, uint64_t smask,
@@ -27,10 +56,11 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
                                            const uint32_t* __restrict__ order,
                                            double* __restrict__ accel, double* __restrict__ pot) {
   __shared__ double4 tile[64];
-  const int lane = threadIdx.x;
+  const uint32_t lane = threadIdx.x;
   const uint32_t leaf = leaf_ids[blockIdx.x];
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
   const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
+  const double4 pad = make_double4(1e100, 1e100, 1e100, 0.0);
   for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
     const uint32_t nt = min(64u, nt_all - t0);
     uint32_t ntp = 1;
@@ -42,34 +72,42 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
     const uint32_t si = ft + t0 + (tvalid ? ti : 0);
     const double xi = bx[si], yi = by[si], zi = bz[si];
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    // 1) the target's own leaf, excluding each body's self term
+    for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
+      __syncthreads();
+      const uint32_t q = ft + s0 + lane;
+      tile[lane] = (s0 + lane < nt_all) ? make_double4(bx[q], by[q], bz[q], bm[q]) : pad;
+      __syncthreads();
+      const int self_j = (int)(t0 + ti) - (int)s0;  // tile index of the lane's own body
+      p2p_tile<true>(tile, sub, K, xi, yi, zi, soft2, self_j, ax, ay, az, ph);
+    }
+    // 2) the other source leaves, packed back to back into full tiles
+    __syncthreads();
+    uint32_t fill = 0;
     for (uint32_t j = jb; j < je; j++) {
       const uint32_t src = (uint32_t)(skeys[j] & smask);
-      const uint32_t fs = cfirst[src], ns_all = ccount[src];
-      for (uint32_t s0 = 0; s0 < ns_all; s0 += 64) {
-        const uint32_t ns = min(64u, ns_all - s0);
-        __syncthreads();
-        if ((uint32_t)lane < ns) {
-          const uint32_t q = fs + s0 + lane;
+      if (src == leaf) continue;
+      const uint32_t fs = cfirst[src], cnt = ccount[src];
+      for (uint32_t off = 0; off < cnt;) {
+        const uint32_t take = min(cnt - off, 64u - fill);
+        if (lane >= fill && lane < fill + take) {
+          const uint32_t q = fs + off + (lane - fill);
           tile[lane] = make_double4(bx[q], by[q], bz[q], bm[q]);
         }
-        __syncthreads();
-        const uint32_t sbase = fs + s0;
-        for (uint32_t q = sub; q < ns; q += K) {
-          const double4 sj = tile[q];
-          const double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
-          const bool self = (sbase + q) == si;
-          double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + soft2;
-          r2 = self ? 1.0 : r2;
-          const double mj = self ? 0.0 : sj.w;
-          const double inv = rsqrt(r2);
-          const double minv = mj * inv;
-          const double s3 = minv * inv * inv;
-          ax = fma(s3, dx, ax);
-          ay = fma(s3, dy, ay);
-          az = fma(s3, dz, az);
-          ph += minv;
+        fill += take;
+        off += take;
+        if (fill == 64) {
+          __syncthreads();
+          p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
+          __syncthreads();
+          fill = 0;
         }
       }
+    }
+    if (fill > 0) {
+      if (lane >= fill) tile[lane] = pad;
+      __syncthreads();
+      p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
     }
     for (uint32_t o = ntp; o < 64; o <<= 1) {
       ax += __shfl_xor(ax, (int)o, 64);
@@ -100,7 +138,8 @@ void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr
 
 __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,
                                                   const uint32_t* __restrict__ count,
-                                                  unsigned long long* __restrict__ out) {
+                                                  unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long sh[4];
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   unsigned long long v = 0;
   if (i < np) {
@@ -109,13 +148,31 @@ __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pai
     v = pr.x == pr.y ? ca * (ca - 1) : 2 * ca * cb;
   }
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+__global__ void __launch_bounds__(256) k_sum_u64(const unsigned long long* __restrict__ part,
+                                                uint32_t np, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long sh[4];
+  unsigned long long v = 0;
+  for (uint32_t i = threadIdx.x; i < np; i += 256) v += part[i];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+size_t p2p_count_scratch(uint32_t np) { return ((size_t)np + 255) / 256 + 1; }
+
+// out[0] = total; out[1..] scratch of p2p_count_scratch(np) words.
 void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
                       unsigned long long* out, hipStream_t s) {
-  if (np == 0) return;
-  hipLaunchKernelGGL(k_p2p_count, dim3((np + 255) / 256), dim3(256), 0, s, pairs, np, count, out);
+  uint32_t nb = (np + 255) / 256;
+  if (nb)
+    hipLaunchKernelGGL(k_p2p_count, dim3(nb), dim3(256), 0, s, pairs, np, count, out + 1);
+  hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, out + 1, nb, out);
 }
 
 // ------------------------------------------------------------ direct sum

@@ -51,14 +51,33 @@ __global__ void __launch_bounds__(TB) k_bbox_partial(const double* __restrict__ 
 }
 
 // tree.rs:99-111: side = max extent (at least f64::MIN_POSITIVE), inv = 1/side.
-__global__ void k_bbox_final(const double* __restrict__ partial, int nb, BBox* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(TB) k_bbox_final(const double* __restrict__ partial, int nb,
+                                                  BBox* __restrict__ out) {
+  __shared__ double sh[6][TB];
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += TB)
     for (int d = 0; d < 3; d++) {
       lo[d] = fmin(lo[d], partial[6 * b + d]);
       hi[d] = fmax(hi[d], partial[6 * b + 3 + d]);
     }
+  for (int d = 0; d < 3; d++) {
+    sh[d][threadIdx.x] = lo[d];
+    sh[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int st = TB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int d = 0; d < 3; d++) {
+        sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + st]);
+        sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < 3; d++) {
+    lo[d] = sh[d][0];
+    hi[d] = sh[3 + d][0];
+  }
   double side = fmax(fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]), 2.2250738585072014e-308);
   for (int d = 0; d < 3; d++) {
     out->lo[d] = lo[d];
@@ -73,7 +92,7 @@ void launch_bbox(const double* pos, size_t n, double* partial, BBox* out, hipStr
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(TB), 0, s, pos, n, partial);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, partial, nb, out);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(TB), 0, s, partial, nb, out);
 }
 
 // Spread the low 21 bits of x to every third bit (tree.rs:28-35).

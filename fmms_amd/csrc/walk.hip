@@ -14,15 +14,33 @@
 
 namespace fmmh {
 
-struct WalkCells {
-  const uint8_t* __restrict__ leaf;
-  const int32_t* __restrict__ children;
-  const double* __restrict__ cx;
-  const double* __restrict__ cy;
-  const double* __restrict__ cz;
-  const double* __restrict__ rho;
-  const double* __restrict__ mass;
+// One 64-byte record per cell with everything a walk decision reads:
+// centre, rho and the child ids (all -1 <=> leaf, since an internal cell
+// always has a non-empty octant).
+struct __align__(64) WalkRec {
+  double cx, cy, cz, rho;
+  int32_t ch[8];
 };
+
+__global__ void __launch_bounds__(256) k_pack_walk(uint32_t nc, DevCells c,
+                                                  WalkRec* __restrict__ rec) {
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  WalkRec r;
+  r.cx = c.cx[i];
+  r.cy = c.cy[i];
+  r.cz = c.cz[i];
+  r.rho = c.rho[i];
+#pragma unroll
+  for (int o = 0; o < 8; o++) r.ch[o] = c.children[(size_t)i * 8 + o];
+  rec[i] = r;
+}
+
+size_t walk_rec_bytes() { return sizeof(WalkRec); }
+
+void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_walk, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, (WalkRec*)rec);
+}
 
 // mac.rs:23-61 error estimate of cell a acting on cell b (r > 0 here).
 __device__ __forceinline__ double error_estimate(uint32_t a, double rho_b, double mass_a,
@@ -44,67 +62,82 @@ __device__ __forceinline__ double error_estimate(uint32_t a, double rho_b, doubl
   return 8.0 * fmax(rho_a, rho_b) / denom * e;
 }
 
-// mac.rs:64-95 (+ theta_max: r2*theta_max^2 <= (rhoA+rhoB)^2 rejects; with
-// theta_max = 1 the product is r2 itself).
-__device__ __forceinline__ bool accept(uint32_t a, uint32_t b, const WalkCells& c,
+// mac.rs:64-95 for both directions of one pair. theta_max extension:
+// r2*theta_max^2 <= (rhoA+rhoB)^2 rejects; theta_max = 1 multiplies by 1.
+__device__ __forceinline__ bool accept(uint32_t a, const WalkRec& ra, uint32_t b,
+                                       const WalkRec& rb, double r2, const double* __restrict__ mass,
                                        const MacParams& mp) {
-  double dx = c.cx[b] - c.cx[a], dy = c.cy[b] - c.cy[a], dz = c.cz[b] - c.cz[a];
-  double r2 = dx * dx + dy * dy + dz * dz;
-  const double ra = c.rho[a], rb = c.rho[b];
-  double sum_rho = ra + rb;
+  double sum_rho = ra.rho + rb.rho;
   if (r2 * mp.theta2 <= sum_rho * sum_rho) return false;
   if (!mp.use_err) return true;
   double r = sqrt(r2);
   if (r == 0.0) return false;
-  const double ma = c.mass[a];
-  double est = error_estimate(a, rb, ma, ra, r, mp);
+  const double ma = mass[a];
+  double est = error_estimate(a, rb.rho, ma, ra.rho, r, mp);
   if (est == 0.0) return true;
   return est * ma / r2 < mp.eps * mp.fmin[b];
 }
 
 enum : int { A_NONE = 0, A_P2P = 1, A_M2L = 2, A_SELF_SPLIT = 3, A_SPLIT_A = 4, A_SPLIT_B = 5 };
 
+__device__ __forceinline__ uint32_t nchild(const WalkRec& r) {
+  uint32_t k = 0;
+#pragma unroll
+  for (int o = 0; o < 8; o++) k += r.ch[o] >= 0 ? 1u : 0u;
+  return k;
+}
+
 __global__ void __launch_bounds__(256) k_walk_round(const uint2* __restrict__ fin, uint32_t nf,
                                                    uint2* __restrict__ fout, uint32_t cap_f,
                                                    uint2* __restrict__ m2l, uint32_t cap_m2l,
                                                    uint2* __restrict__ p2p, uint32_t cap_p2p,
-                                                   WalkCounters* __restrict__ ctr, WalkCells c,
-                                                   MacParams mp) {
+                                                   WalkCounters* __restrict__ ctr,
+                                                   const WalkRec* __restrict__ rec,
+                                                   const double* __restrict__ mass, MacParams mp) {
   const int lane = threadIdx.x & 63;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < nf;
   uint32_t a = 0, b = 0;
   int act = A_NONE;
   uint32_t nch = 0;
+  WalkRec ra, rb;
   if (valid) {
     uint2 pr = fin[t];
     a = pr.x;
     b = pr.y;
-    const bool la = c.leaf[a], lb = c.leaf[b];
+    ra = rec[a];
+    rb = rec[b];
+    const uint32_t na = nchild(ra), nb = nchild(rb);
+    const bool la = na == 0, lb = nb == 0;
     if (a == b) {
       if (la) {
         act = A_P2P;
       } else {
         act = A_SELF_SPLIT;
-        uint32_t k = 0;
-        for (int o = 0; o < 8; o++) k += c.children[(size_t)a * 8 + o] >= 0 ? 1u : 0u;
-        nch = k * (k + 1) / 2;
+        nch = na * (na + 1) / 2;
       }
-    } else if (accept(a, b, c, mp) && accept(b, a, c, mp)) {
-      act = A_M2L;
-    } else if (la && lb) {
-      act = A_P2P;
     } else {
-      bool split_b;
-      if (la)
-        split_b = true;
-      else if (lb)
-        split_b = false;
-      else
-        split_b = !(c.rho[a] >= c.rho[b]);
-      act = split_b ? A_SPLIT_B : A_SPLIT_A;
-      uint32_t s = split_b ? b : a;
-      for (int o = 0; o < 8; o++) nch += c.children[(size_t)s * 8 + o] >= 0 ? 1u : 0u;
+      // walk.rs:138-139: accept(a,b) && accept(b,a); mac.rs:75-78 forms
+      // r = cen_b - cen_a for the first call and cen_a - cen_b for the second.
+      double dx = rb.cx - ra.cx, dy = rb.cy - ra.cy, dz = rb.cz - ra.cz;
+      double r2ab = dx * dx + dy * dy + dz * dz;
+      double ex = ra.cx - rb.cx, ey = ra.cy - rb.cy, ez = ra.cz - rb.cz;
+      double r2ba = ex * ex + ey * ey + ez * ez;
+      if (accept(a, ra, b, rb, r2ab, mass, mp) && accept(b, rb, a, ra, r2ba, mass, mp)) {
+        act = A_M2L;
+      } else if (la && lb) {
+        act = A_P2P;
+      } else {
+        bool split_b;
+        if (la)
+          split_b = true;
+        else if (lb)
+          split_b = false;
+        else
+          split_b = !(ra.rho >= rb.rho);
+        act = split_b ? A_SPLIT_B : A_SPLIT_A;
+        nch = split_b ? nb : na;
+      }
     }
   }
   // --- child pairs: wave-aggregated reservation in the next frontier
@@ -118,20 +151,17 @@ __global__ void __launch_bounds__(256) k_walk_round(const uint2* __restrict__ fi
     if (w + nch > cap_f) {
       atomicOr(&ctr->overflow, 1u);
     } else if (act == A_SELF_SPLIT) {
-      const int32_t* ch = c.children + (size_t)a * 8;
       for (int i = 0; i < 8; i++) {
-        if (ch[i] < 0) continue;
+        if (ra.ch[i] < 0) continue;
         for (int j = i; j < 8; j++)
-          if (ch[j] >= 0) fout[w++] = make_uint2((uint32_t)ch[i], (uint32_t)ch[j]);
+          if (ra.ch[j] >= 0) fout[w++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
       }
     } else if (act == A_SPLIT_B) {
-      const int32_t* ch = c.children + (size_t)b * 8;
       for (int j = 0; j < 8; j++)
-        if (ch[j] >= 0) fout[w++] = make_uint2(a, (uint32_t)ch[j]);
+        if (rb.ch[j] >= 0) fout[w++] = make_uint2(a, (uint32_t)rb.ch[j]);
     } else {
-      const int32_t* ch = c.children + (size_t)a * 8;
       for (int i = 0; i < 8; i++)
-        if (ch[i] >= 0) fout[w++] = make_uint2((uint32_t)ch[i], b);
+        if (ra.ch[i] >= 0) fout[w++] = make_uint2((uint32_t)ra.ch[i], b);
     }
   }
   // --- M2L and P2P appends
@@ -166,10 +196,9 @@ __global__ void __launch_bounds__(256) k_walk_round(const uint2* __restrict__ fi
 
 void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
                        uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
-                       DevCells c, MacParams mac, hipStream_t s) {
-  WalkCells wc{c.leaf, c.children, c.cx, c.cy, c.cz, c.rho, c.mass};
+                       const void* rec, const double* mass, MacParams mac, hipStream_t s) {
   hipLaunchKernelGGL(k_walk_round, dim3((nf + 255) / 256), dim3(256), 0, s, fin, nf, fout, cap_f,
-                     m2l, cap_m2l, p2p, cap_p2p, ctr, wc, mac);
+                     m2l, cap_m2l, p2p, cap_p2p, ctr, (const WalkRec*)rec, mass, mac);
 }
 
 // Directed M2L entries (target << sbits | source), both directions per pair.

@@ -204,9 +204,13 @@ size_t build_rec(const std::vector<uint64_t>& keys, std::vector<Cell>& cells, ui
 // --------------------------------------------------------------- harmonic.rs
 // harmonic.rs:91-129. For n = m+1 the (n-2, m) read lands on slot m^2 (the
 // not-yet-written Theta_m^{-m}) and is multiplied by zero, as in the reference.
-void theta_into(const double r[3], int p, C* t) {
+void theta_into(const double r[3], int p, C* t, double eps2 = 0.0) {
   double x = r[0], y = r[1], z = r[2];
   double r2 = x * x + y * y + z * z;
+  // Softening extension: the Plummer kernel (r^2+eps^2)^(-1/2) is a function
+  // of r^2+eps^2, so the same recursion with r^2 -> r^2+eps^2 yields its
+  // traceless derivatives. eps2 == 0 leaves the reference arithmetic intact.
+  if (eps2 > 0.0) r2 = r2 + eps2;
   double ir = 1.0 / std::sqrt(r2);
   t[0] = C{ir, 0.0};
   for (int n = 1; n <= p; n++) {
@@ -350,10 +354,10 @@ struct M2lScratch {
 };
 
 void m2l_mutual(int p, const double* cena, const double* cenb, const C* ma, const C* mb, C* la,
-                C* lb, M2lScratch& s) {
+                C* lb, M2lScratch& s, double eps2 = 0.0) {
   size_t d = dimp(p);
   double r[3] = {cenb[0] - cena[0], cenb[1] - cena[1], cenb[2] - cena[2]};
-  theta_into(r, p, s.th.data());
+  theta_into(r, p, s.th.data(), eps2);
   for (size_t i = 0; i < d; i++) {
     s.cma[i] = cconj(ma[i]);
     s.cmb[i] = cconj(mb[i]);
@@ -537,17 +541,29 @@ struct RotCtx {  // rotation.rs:106-253
   }
 
   // rotation.rs:262-301
-  void m2l_pair(const double* cena, const double* cenb, const C* ma, const C* mb, C* la, C* lb) {
+  void m2l_pair(const double* cena, const double* cenb, const C* ma, const C* mb, C* la, C* lb,
+                double eps2 = 0.0) {
     double r[3] = {cenb[0] - cena[0], cenb[1] - cena[1], cenb[2] - cena[2]};
     double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
     double rinv = 1.0 / std::sqrt(r2);
     double alpha = std::atan2(r[1], r[0]);
     double s_beta = std::sqrt(r[0] * r[0] + r[1] * r[1]);
     double beta = std::atan2(s_beta, r[2]);
-    double irp = rinv;
-    for (int j = 0; j <= p; j++) {
-      facr[j] = fact[j] * irp;
-      irp *= rinv;
+    if (eps2 > 0.0) {
+      // softening extension: axial Theta_j^0(0,0,R) of the Plummer kernel
+      // by the theta recursion with R^2 -> R^2 + eps^2 (see theta_into).
+      double R = std::sqrt(r2), s2 = r2 + eps2;
+      facr[0] = 1.0 / std::sqrt(s2);
+      for (int j = 1; j <= p; j++) {
+        double prev2 = j >= 2 ? facr[j - 2] : 0.0;
+        facr[j] = ((double)(2 * j - 1) * R * facr[j - 1] - (double)((j - 1) * (j - 1)) * prev2) / s2;
+      }
+    } else {
+      double irp = rinv;
+      for (int j = 0; j <= p; j++) {
+        facr[j] = fact[j] * irp;
+        irp *= rinv;
+      }
     }
     for (int m = -p; m <= p; m++) {
       double a = (double)m * alpha;
@@ -1293,9 +1309,9 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
         t->m2l.push_back((uint32_t)a);
         t->m2l.push_back((uint32_t)b);
         if (rot)
-          rot->m2l_pair(ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d);
+          rot->m2l_pair(ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, soft2);
         else
-          m2l_mutual(p, ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, scr);
+          m2l_mutual(p, ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, scr, soft2);
       } else if (act == ACT_P2P) {
         t->p2p.push_back((uint32_t)a);
         t->p2p.push_back((uint32_t)b);
@@ -1404,9 +1420,9 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
         const Cell& ca = t->cells[a];
         const Cell& cb = t->cells[b];
         if (rot)
-          rot->m2l_pair(ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d);
+          rot->m2l_pair(ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, soft2);
         else
-          m2l_mutual(p, ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, scr);
+          m2l_mutual(p, ca.cen, cb.cen, mom + a * d, mom + b * d, loc + a * d, loc + b * d, scr, soft2);
       }
       delete rot;
     });
