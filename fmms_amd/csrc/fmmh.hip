@@ -83,7 +83,7 @@ struct fmmh_ctx {
   // buffers
   Buf in_pos, in_mass, out_acc, out_pot;
   Buf keys, keys_alt, vals, vals_alt;
-  Buf bx, by, bz, bm;
+  Buf bx, by, bz, bm, bxyzm, prange;
   Buf bbox_partial, bbox, flags;
   Buf chain, top, cstart;
   Buf bfs_first, bfs_count, bfs_parent, bfs_oct, bfs_level, bfs_dfs, nchild, child_off;
@@ -101,7 +101,7 @@ struct fmmh_ctx {
 
   fmmh_ctx() {
     Buf* all[] = {&in_pos, &in_mass, &out_acc, &out_pot, &keys, &keys_alt, &vals, &vals_alt,
-                  &bx, &by, &bz, &bm, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
+                  &bx, &by, &bz, &bm, &bxyzm, &prange, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
@@ -190,7 +190,9 @@ struct fmmh_ctx {
                     (double*)c_mass.p,    (double*)c_rho.p,    (int32_t*)c_rope.p,
                     (int32_t*)c_last.p};
   }
-  DevBodies bodies() { return DevBodies{(double*)bx.p, (double*)by.p, (double*)bz.p, (double*)bm.p}; }
+  DevBodies bodies() {
+    return DevBodies{(double*)bx.p, (double*)by.p, (double*)bz.p, (double*)bm.p, (double4*)bxyzm.p};
+  }
   DevBfs bfs() {
     return DevBfs{(uint32_t*)bfs_first.p, (uint32_t*)bfs_count.p, (int32_t*)bfs_parent.p,
                   (uint8_t*)bfs_oct.p,    (uint8_t*)bfs_level.p,  (uint32_t*)bfs_dfs.p};
@@ -223,6 +225,7 @@ struct fmmh_ctx {
     ensure(by, n * 8);
     ensure(bz, n * 8);
     ensure(bm, n * 8);
+    ensure(bxyzm, n * 32);
     ensure(bbox_partial, 6 * 1024 * 8);
     ensure(bbox, sizeof(BBox));
     ensure(flags, 64);
@@ -410,8 +413,10 @@ struct fmmh_ctx {
     ensure(far, n * sizeof(double4));
     launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
     mark(ST_P2P);
-    launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, sbits, c, b,
-               prm.softening * prm.softening, (double4*)far.p, order, acc, pot, s);
+    ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
+    launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
+               b.xyzm, (uint2*)prange.p, prm.softening * prm.softening, (double4*)far.p, order,
+               acc, pot, s);
     mark(ST_COUNT);
     p2p_interactions = read1((const unsigned long long*)p2p_work.p);
     HCHECK(hipGetLastError());

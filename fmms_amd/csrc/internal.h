@@ -41,6 +41,7 @@ struct DevBodies {  // slot (Morton) order, bodies.rs:13-22
   double* y;
   double* z;
   double* m;
+  double4* xyzm;  // the same bodies interleaved (one 32-byte load per body)
 };
 
 // BFS (level-order) scratch of the tree build.
@@ -136,9 +137,9 @@ bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, Dev
 
 // -------------------------------------------------------------- near.hip
 void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
-                const double4* far, const uint32_t* order, double* accel, double* pot,
-                hipStream_t s);
+                const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, const double4* body,
+                uint2* prange, double soft2, const double4* far, const uint32_t* order,
+                double* accel, double* pot, hipStream_t s);
 // Directed body-body interactions of a P2P pair list (self pairs c(c-1));
 // out[0] = total, out needs 1 + p2p_count_scratch(np) words.
 size_t p2p_count_scratch(uint32_t np);

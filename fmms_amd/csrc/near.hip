@@ -43,19 +43,19 @@ __device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint3
   }
 }
 
+constexpr uint32_t P2P_RMAX = 256;  // source ranges staged in LDS per pass
+
 __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_ids,
                                            const uint32_t* __restrict__ pptr,
-                                           const uint64_t* __restrict__ skeys, uint64_t smask,
+                                           const uint2* __restrict__ prange,
                                            const uint32_t* __restrict__ cfirst,
                                            const uint32_t* __restrict__ ccount,
-                                           const double* __restrict__ bx,
-                                           const double* __restrict__ by,
-                                           const double* __restrict__ bz,
-                                           const double* __restrict__ bm, double soft2,
+                                           const double4* __restrict__ body, double soft2,
                                            const double4* __restrict__ far,
                                            const uint32_t* __restrict__ order,
                                            double* __restrict__ accel, double* __restrict__ pot) {
   __shared__ double4 tile[64];
+  __shared__ uint2 rng[P2P_RMAX];
   const uint32_t lane = threadIdx.x;
   const uint32_t leaf = leaf_ids[blockIdx.x];
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
@@ -70,44 +70,57 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
     const uint32_t sub = lane / ntp;
     const bool tvalid = ti < nt;
     const uint32_t si = ft + t0 + (tvalid ? ti : 0);
-    const double xi = bx[si], yi = by[si], zi = bz[si];
+    const double4 me = body[si];
+    const double xi = me.x, yi = me.y, zi = me.z;
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
     // 1) the target's own leaf, excluding each body's self term
     for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
       __syncthreads();
-      const uint32_t q = ft + s0 + lane;
-      tile[lane] = (s0 + lane < nt_all) ? make_double4(bx[q], by[q], bz[q], bm[q]) : pad;
+      tile[lane] = (s0 + lane < nt_all) ? body[ft + s0 + lane] : pad;
       __syncthreads();
-      const int self_j = (int)(t0 + ti) - (int)s0;  // tile index of the lane's own body
+      const int self_j = (int)(t0 + ti) - (int)s0;
       p2p_tile<true>(tile, sub, K, xi, yi, zi, soft2, self_j, ax, ay, az, ph);
     }
-    // 2) the other source leaves, packed back to back into full tiles
-    __syncthreads();
-    uint32_t fill = 0;
-    for (uint32_t j = jb; j < je; j++) {
-      const uint32_t src = (uint32_t)(skeys[j] & smask);
-      if (src == leaf) continue;
-      const uint32_t fs = cfirst[src], cnt = ccount[src];
-      for (uint32_t off = 0; off < cnt;) {
-        const uint32_t take = min(cnt - off, 64u - fill);
-        if (lane >= fill && lane < fill + take) {
-          const uint32_t q = fs + off + (lane - fill);
-          tile[lane] = make_double4(bx[q], by[q], bz[q], bm[q]);
-        }
-        fill += take;
-        off += take;
-        if (fill == 64) {
-          __syncthreads();
-          p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
-          __syncthreads();
-          fill = 0;
-        }
-      }
-    }
-    if (fill > 0) {
-      if (lane >= fill) tile[lane] = pad;
+    // 2) the other source leaves: their slot ranges are staged in LDS, then
+    // packed back to back into full 64-body tiles. The global loads of the
+    // next tile are issued before the current tile is computed, so their
+    // latency hides behind the arithmetic.
+    for (uint32_t seg = jb; seg < je; seg += P2P_RMAX) {
+      const uint32_t nr = min(P2P_RMAX, je - seg);
       __syncthreads();
-      p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
+      for (uint32_t q = lane; q < nr; q += 64) {
+        uint2 g = prange[seg + q];
+        if (g.x == ft) g.y = 0;  // the own leaf was done above
+        rng[q] = g;
+      }
+      __syncthreads();
+      uint32_t r = 0, off = 0;
+      auto gather = [&](double4* v) -> bool {
+        uint32_t fill = 0;
+        int slot = -1;
+        while (fill < 64 && r < nr) {
+          const uint2 g = rng[r];
+          const uint32_t take = min(g.y - off, 64u - fill);
+          if (lane >= fill && lane < fill + take) slot = (int)(g.x + off + (lane - fill));
+          fill += take;
+          off += take;
+          if (off == g.y) {
+            r++;
+            off = 0;
+          }
+        }
+        *v = slot >= 0 ? body[slot] : pad;
+        return fill > 0;
+      };
+      double4 cur;
+      bool have = gather(&cur);
+      while (have) {
+        tile[lane] = cur;
+        __syncthreads();
+        have = gather(&cur);
+        p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
+        __syncthreads();
+      }
     }
     for (uint32_t o = ntp; o < 64; o <<= 1) {
       ax += __shfl_xor(ax, (int)o, 64);
@@ -126,14 +139,29 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
   }
 }
 
+// prange[j] = (first, count) of the source leaf of directed entry j.
+__global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__ skeys,
+                                                   uint32_t nnz, uint64_t smask,
+                                                   const uint32_t* __restrict__ cfirst,
+                                                   const uint32_t* __restrict__ ccount,
+                                                   uint2* __restrict__ prange) {
+  uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= nnz) return;
+  const uint32_t src = (uint32_t)(skeys[j] & smask);
+  prange[j] = make_uint2(cfirst[src], ccount[src]);
+}
+
 void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                const uint64_t* skeys, int sbits, DevCells c, DevBodies b, double soft2,
-                const double4* far, const uint32_t* order, double* accel, double* pot,
-                hipStream_t s) {
+                const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, const double4* body,
+                uint2* prange, double soft2, const double4* far, const uint32_t* order,
+                double* accel, double* pot, hipStream_t s) {
   if (nleaves == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_p2p, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, skeys, smask, c.first,
-                     c.count, b.x, b.y, b.z, b.m, soft2, far, order, accel, pot);
+  if (nnz)
+    hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
+                       c.first, c.count, prange);
+  hipLaunchKernelGGL(k_p2p, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, prange, c.first,
+                     c.count, body, soft2, far, order, accel, pot);
 }
 
 __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,

@@ -148,10 +148,12 @@ __global__ void __launch_bounds__(TB) k_gather(const double* __restrict__ pos,
   size_t k = (size_t)blockIdx.x * TB + threadIdx.x;
   if (k >= n) return;
   size_t id = order[k];
-  b.x[k] = pos[3 * id];
-  b.y[k] = pos[3 * id + 1];
-  b.z[k] = pos[3 * id + 2];
-  b.m[k] = mass[id];
+  const double x = pos[3 * id], y = pos[3 * id + 1], z = pos[3 * id + 2], m = mass[id];
+  b.x[k] = x;
+  b.y[k] = y;
+  b.z[k] = z;
+  b.m[k] = m;
+  b.xyzm[k] = make_double4(x, y, z, m);
 }
 
 void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,

@@ -87,7 +87,9 @@ __device__ __forceinline__ uint32_t nchild(const WalkRec& r) {
   return k;
 }
 
-__global__ void __launch_bounds__(256) k_walk_round(const uint2* __restrict__ fin, uint32_t nf,
+constexpr int WALK_T = 1024;
+
+__global__ void __launch_bounds__(WALK_T) k_walk_round(const uint2* __restrict__ fin, uint32_t nf,
                                                    uint2* __restrict__ fout, uint32_t cap_f,
                                                    uint2* __restrict__ m2l, uint32_t cap_m2l,
                                                    uint2* __restrict__ p2p, uint32_t cap_p2p,
@@ -140,64 +142,72 @@ __global__ void __launch_bounds__(256) k_walk_round(const uint2* __restrict__ fi
       }
     }
   }
-  // --- child pairs: wave-aggregated reservation in the next frontier
+  // --- appends, aggregated per block: one atomic per counter per block
+  __shared__ uint32_t wsum[WALK_T / 64][3];
+  __shared__ uint32_t wbase[WALK_T / 64][3];
+  const int w = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  const uint64_t bm = __ballot(act == A_M2L);
+  const uint64_t bp = __ballot(act == A_P2P);
   uint32_t wtot;
-  uint32_t pre = wave_excl_scan(nch, &wtot);
-  uint32_t base = 0;
-  if (lane == 0 && wtot) base = atomicAdd(&ctr->next, wtot);
-  base = __shfl(base, 0);
+  const uint32_t pre = wave_excl_scan(nch, &wtot);
+  if (lane == 0) {
+    wsum[w][0] = wtot;
+    wsum[w][1] = (uint32_t)__popcll(bm);
+    wsum[w][2] = (uint32_t)__popcll(bp);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    uint32_t tot = 0;
+    for (int q = 0; q < WALK_T / 64; q++) {
+      wbase[q][k] = tot;
+      tot += wsum[q][k];
+    }
+    uint32_t* ctrk = k == 0 ? &ctr->next : (k == 1 ? &ctr->m2l : &ctr->p2p);
+    const uint32_t b0 = tot ? atomicAdd(ctrk, tot) : 0u;
+    for (int q = 0; q < WALK_T / 64; q++) wbase[q][k] += b0;
+  }
+  __syncthreads();
   if (nch) {
-    uint32_t w = base + pre;
-    if (w + nch > cap_f) {
+    uint32_t wp = wbase[w][0] + pre;
+    if (wp + nch > cap_f) {
       atomicOr(&ctr->overflow, 1u);
     } else if (act == A_SELF_SPLIT) {
       for (int i = 0; i < 8; i++) {
         if (ra.ch[i] < 0) continue;
         for (int j = i; j < 8; j++)
-          if (ra.ch[j] >= 0) fout[w++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
+          if (ra.ch[j] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
       }
     } else if (act == A_SPLIT_B) {
       for (int j = 0; j < 8; j++)
-        if (rb.ch[j] >= 0) fout[w++] = make_uint2(a, (uint32_t)rb.ch[j]);
+        if (rb.ch[j] >= 0) fout[wp++] = make_uint2(a, (uint32_t)rb.ch[j]);
     } else {
       for (int i = 0; i < 8; i++)
-        if (ra.ch[i] >= 0) fout[w++] = make_uint2((uint32_t)ra.ch[i], b);
+        if (ra.ch[i] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], b);
     }
   }
-  // --- M2L and P2P appends
-  const uint64_t lt = lanemask_lt();
-  uint64_t bm = __ballot(act == A_M2L);
-  if (bm) {
-    uint32_t mb = 0;
-    if (lane == 0) mb = atomicAdd(&ctr->m2l, (uint32_t)__popcll(bm));
-    mb = __shfl(mb, 0);
-    if (act == A_M2L) {
-      uint32_t w = mb + (uint32_t)__popcll(bm & lt);
-      if (w < cap_m2l)
-        m2l[w] = make_uint2(a, b);
-      else
-        atomicOr(&ctr->overflow, 2u);
-    }
+  if (act == A_M2L) {
+    const uint32_t wp = wbase[w][1] + (uint32_t)__popcll(bm & lt);
+    if (wp < cap_m2l)
+      m2l[wp] = make_uint2(a, b);
+    else
+      atomicOr(&ctr->overflow, 2u);
   }
-  uint64_t bp = __ballot(act == A_P2P);
-  if (bp) {
-    uint32_t pb = 0;
-    if (lane == 0) pb = atomicAdd(&ctr->p2p, (uint32_t)__popcll(bp));
-    pb = __shfl(pb, 0);
-    if (act == A_P2P) {
-      uint32_t w = pb + (uint32_t)__popcll(bp & lt);
-      if (w < cap_p2p)
-        p2p[w] = make_uint2(a, b);
-      else
-        atomicOr(&ctr->overflow, 4u);
-    }
+  if (act == A_P2P) {
+    const uint32_t wp = wbase[w][2] + (uint32_t)__popcll(bp & lt);
+    if (wp < cap_p2p)
+      p2p[wp] = make_uint2(a, b);
+    else
+      atomicOr(&ctr->overflow, 4u);
   }
 }
 
 void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
                        uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
                        const void* rec, const double* mass, MacParams mac, hipStream_t s) {
-  hipLaunchKernelGGL(k_walk_round, dim3((nf + 255) / 256), dim3(256), 0, s, fin, nf, fout, cap_f,
+  hipLaunchKernelGGL(k_walk_round, dim3((nf + WALK_T - 1) / WALK_T), dim3(WALK_T), 0, s, fin, nf,
+                     fout, cap_f,
                      m2l, cap_m2l, p2p, cap_p2p, ctr, (const WalkRec*)rec, mass, mac);
 }
 
