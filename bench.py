@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device (a copy kernel of known "
+                         "size, for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -184,6 +187,15 @@ def main():
         acc_cpu, cpu = cpu_baseline(pos, mass, args, n_leaf)
         d = np.linalg.norm(acc - acc_cpu, axis=1) / np.linalg.norm(acc_cpu, axis=1)
         cpu["max_rel_diff_vs_gpu"] = float(d.max())
+
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src = hip.DeviceArray((nb // 8,))
+        dst = hip.DeviceArray((nb // 8,))
+        dst.copy_from(src)
+        hip.synchronize()
+        src.free()
+        dst.free()
 
     out = {
         "metric": METRIC,

@@ -45,6 +45,36 @@ __device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint3
 
 constexpr uint32_t P2P_RMAX = 256;  // source ranges staged in LDS per pass
 
+struct TileLoad {
+  double4 v;
+  uint32_t fill;
+};
+
+// Next tile of the packed source stream: advances the (range, offset)
+// cursor over up to 64 bodies; the lane's body is loaded (not yet waited on).
+__device__ __forceinline__ TileLoad gather_tile(const uint2* rng, uint32_t nr, uint32_t& r,
+                                                uint32_t& off, uint32_t lane,
+                                                const double4* __restrict__ body, double4 pad) {
+  uint32_t fill = 0;
+  int slot = -1;
+  while (fill < 64 && r < nr) {
+    const uint2 g = rng[r];
+    const uint32_t take = min(g.y - off, 64u - fill);
+    if (lane >= fill && lane < fill + take) slot = (int)(g.x + off + (lane - fill));
+    fill += take;
+    off += take;
+    if (off == g.y) {
+      r++;
+      off = 0;
+    }
+  }
+  TileLoad t;
+  t.v = body[slot >= 0 ? slot : 0];  // in-bounds load, then padded by value
+  if (slot < 0) t.v = pad;
+  t.fill = fill;
+  return t;
+}
+
 __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_ids,
                                            const uint32_t* __restrict__ pptr,
                                            const uint2* __restrict__ prange,
@@ -76,7 +106,10 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
     // 1) the target's own leaf, excluding each body's self term
     for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
       __syncthreads();
-      tile[lane] = (s0 + lane < nt_all) ? body[ft + s0 + lane] : pad;
+      const bool in = s0 + lane < nt_all;
+      double4 v = body[in ? ft + s0 + lane : ft];
+      if (!in) v = pad;
+      tile[lane] = v;
       __syncthreads();
       const int self_j = (int)(t0 + ti) - (int)s0;
       p2p_tile<true>(tile, sub, K, xi, yi, zi, soft2, self_j, ax, ay, az, ph);
@@ -95,29 +128,11 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
       }
       __syncthreads();
       uint32_t r = 0, off = 0;
-      auto gather = [&](double4* v) -> bool {
-        uint32_t fill = 0;
-        int slot = -1;
-        while (fill < 64 && r < nr) {
-          const uint2 g = rng[r];
-          const uint32_t take = min(g.y - off, 64u - fill);
-          if (lane >= fill && lane < fill + take) slot = (int)(g.x + off + (lane - fill));
-          fill += take;
-          off += take;
-          if (off == g.y) {
-            r++;
-            off = 0;
-          }
-        }
-        *v = slot >= 0 ? body[slot] : pad;
-        return fill > 0;
-      };
-      double4 cur;
-      bool have = gather(&cur);
-      while (have) {
-        tile[lane] = cur;
+      TileLoad cur = gather_tile(rng, nr, r, off, lane, body, pad);
+      while (cur.fill > 0) {
+        tile[lane] = cur.v;
         __syncthreads();
-        have = gather(&cur);
+        cur = gather_tile(rng, nr, r, off, lane, body, pad);
         p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
         __syncthreads();
       }

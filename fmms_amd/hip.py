@@ -28,7 +28,7 @@ def runtime():
     return _hip
 
 
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def _check(rc, what):
@@ -72,6 +72,11 @@ class DeviceArray:
         out = np.empty(self.shape, self.dtype)
         _check(runtime().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of other's bytes into this buffer."""
+        n = min(self.nbytes, other.nbytes)
+        _check(runtime().hipMemcpy(self.ptr, other.ptr, n, D2D), "hipMemcpy D2D")
 
     def free(self):
         if self.ptr:
