@@ -51,6 +51,22 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
   return x - v;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8). Remapping gives each XCD a contiguous range of logical blocks, so
+// neighbouring work (which shares source data) shares one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, i = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return x * per + min(x, rem) + i;
+}
+
+// 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
+// second-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2.
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 struct BBox {
   double lo[3];
   double hi[3];

@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(64) k_m2l(const M2lChunk* __restrict__ chunks,
                                            const cplx* __restrict__ moments, double soft2,
                                            cplx* __restrict__ part) {
   constexpr int D = hdim(P), NF = hnpos(P);
-  const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t ci = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (ci >= nchunks) return;
   const M2lChunk ck = chunks[ci];
   const double tx = ccx[ck.target], ty = ccy[ck.target], tz = ccz[ck.target];

@@ -26,14 +26,15 @@ __device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint3
   for (uint32_t j = sub; j < 64; j += K) {
     const double4 sj = tile[j];
     const double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
-    double r2 = fma(dx, dx, fma(dy, dy, dz * dz)) + soft2;
+    double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft2)));
     double mj = sj.w;
     if (SELF) {
       const bool self = (int)j == self_j;
       r2 = self ? 1.0 : r2;
       mj = self ? 0.0 : mj;
     }
-    const double inv = rsqrt(r2);
+    // r2 is finite and > 0 here (softening, or distinct bodies, or padding)
+    const double inv = rsqrt_pos(r2);
     const double minv = mj * inv;
     const double s3 = minv * inv * inv;
     ax = fma(s3, dx, ax);
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
   __shared__ double4 tile[64];
   __shared__ uint2 rng[P2P_RMAX];
   const uint32_t lane = threadIdx.x;
-  const uint32_t leaf = leaf_ids[blockIdx.x];
+  const uint32_t leaf = leaf_ids[xcd_block(blockIdx.x, gridDim.x)];
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
   const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
   const double4 pad = make_double4(1e100, 1e100, 1e100, 0.0);
