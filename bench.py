@@ -169,6 +169,16 @@ def main():
             "frac": round(stage_roof[dom]["frac"], 4), "traffic": None,
             "work_per_launch": kernels[dom][1], "avg_launch_ms": round(stage_roof[dom]["ms"], 4)}
 
+    # HBM bytes per launch of the dominant kernel, measured by rocprofv3
+    # FETCH_SIZE/WRITE_SIZE runs of this workload (profiles/), when present.
+    tf = os.path.join(ROOT, "profiles", "traffic_plummer1M_p8_theta0.5_fp64.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf))
+        if (t.get("n_leaf") == n_leaf and t.get("workload") == f"plummer_N{n}_p{args.p}_theta{args.theta}"
+                f"_soft{args.softening}_fp64" and dom in t.get("bytes_per_launch", {})):
+            roof["traffic"] = t["bytes_per_launch"][dom]
+            roof["traffic_source"] = t["source"]
+
     err = None
     if not args.no_error:
         ids = fa.sample_ids(n, args.samples, 7)
