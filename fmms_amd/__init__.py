@@ -34,7 +34,7 @@ FMMH_E_NONFINITE = 5
 FMMH_E_HIP = 6
 FMMH_E_STATE = 7
 FMMH_FLAG_TIMING = 1
-MAX_P = 12
+MAX_P = 40
 
 _CODES = {1: "EMPTY", 2: "PARAM", 3: "OVER_BUDGET", 4: "OVERFULL_LEAF", 5: "NONFINITE", 6: "HIP",
           7: "STATE"}

@@ -41,6 +41,8 @@ SOURCES = {
     "m2l.hip": "fast",
     "downward.hip": "fast",
     "near.hip": "fast",
+    "generic_up.hip": "off",
+    "generic_far.hip": "fast",
     "fmmh.hip": "off",
 }
 

@@ -96,6 +96,9 @@ struct fmmh_ctx {
   Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
+  Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
+  int rot_p = -1;
+  std::vector<char> rot_struct;
   Buf direct_ids, direct_out;
   Buf pinned;
 
@@ -108,7 +111,8 @@ struct fmmh_ctx {
                   &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &direct_ids, &direct_out};
+                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
+                  &rt_fact, &direct_ids, &direct_out};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -319,6 +323,7 @@ struct fmmh_ctx {
   void evaluate(const double* pos, const double* mass, double* acc, double* pot, bool regather) {
     const int p = prm.p;
     const size_t D = (size_t)hdim(p);
+    rot_struct.resize(std::max(rot_struct.size(), rot_tables_struct_bytes()));
     DevCells c = cells();
     DevBodies b = bodies();
     if (regather) {
@@ -328,9 +333,18 @@ struct fmmh_ctx {
     mark(ST_UPWARD);
     cplx* mom = ensure<cplx>(moments, nc * D * sizeof(cplx));
     cplx* loc = ensure<cplx>(locals, nc * D * sizeof(cplx));
-    launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
-    for (int L = (int)max_level - 1; L >= 0; L--)
-      launch_m2m(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom, s);
+    const bool gen = p > TEMPLATED_MAX_P;
+    if (!gen) {
+      launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
+      for (int L = (int)max_level - 1; L >= 0; L--)
+        launch_m2m(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom, s);
+    } else {
+      HCHECK(hipMemsetAsync(mom, 0, nc * D * sizeof(cplx), s));
+      launch_p2m_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
+      for (int L = (int)max_level - 1; L >= 0; L--)
+        launch_m2m_gen(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom,
+                       s);
+    }
     mark(ST_PREPASS);
     const bool use_err = std::isfinite(prm.eps_mac);
     {
@@ -385,33 +399,50 @@ struct fmmh_ctx {
     uint32_t nself = read1((const uint32_t*)pcount.p);
     uint32_t pnnz = 2 * n_p2p - nself;
     launch_csr_ptr(psorted, pnnz, sbits, nc, (uint32_t*)pptr.p, s);
-    // M2L chunks
-    size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
-    ensure(nch, ((size_t)nc + 1) * 4);
-    ensure(choff, ((size_t)nc + 1) * 4);
-    ensure(chunks, maxck * m2l_chunk_bytes());
-    ensure_scan(nc + 1);
-    uint32_t* tot = nullptr;
-    m2l_build_chunks((uint32_t*)tptr.p, nc, (uint32_t*)nch.p, (uint32_t*)choff.p,
-                     (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
-    // choff[nc] = total chunks (the reduce reads choff[t+1])
-    HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
-    uint32_t nchunks = read1(tot);
-    ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
     ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
     launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
                      (unsigned long long*)p2p_work.p, s);
-    mark(ST_M2L);
-    launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, prm.softening * prm.softening,
-                      (cplx*)part.p, s);
-    mark(ST_M2L_REDUCE);
-    launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
+    if (!gen) {
+      // M2L chunks
+      size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
+      ensure(nch, ((size_t)nc + 1) * 4);
+      ensure(choff, ((size_t)nc + 1) * 4);
+      ensure(chunks, maxck * m2l_chunk_bytes());
+      ensure_scan(nc + 1);
+      uint32_t* tot = nullptr;
+      m2l_build_chunks((uint32_t*)tptr.p, nc, (uint32_t*)nch.p, (uint32_t*)choff.p,
+                       (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
+      // choff[nc] = total chunks (the reduce reads choff[t+1])
+      HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
+      uint32_t nchunks = read1(tot);
+      ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
+      mark(ST_M2L);
+      launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, prm.softening * prm.softening,
+                        (cplx*)part.p, s);
+      mark(ST_M2L_REDUCE);
+      launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
+    } else {
+      if (p >= 20) rot_tables();
+      mark(ST_M2L);
+      launch_m2l_gen(p, nc, (uint32_t*)tptr.p, msorted, sbits, c, mom,
+                     prm.softening * prm.softening, rot_struct.data(), loc, s);
+      mark(ST_M2L_REDUCE);
+    }
     mark(ST_L2L);
-    for (int L = 1; L <= (int)max_level; L++)
-      launch_l2l(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, loc, s);
+    for (int L = 1; L <= (int)max_level; L++) {
+      const uint32_t* lc = (uint32_t*)lvl_cells.p + lvl_off[L];
+      const uint32_t cnt = lvl_off[L + 1] - lvl_off[L];
+      if (!gen)
+        launch_l2l(p, lc, cnt, c, loc, s);
+      else
+        launch_l2l_gen(p, lc, cnt, c, loc, s);
+    }
     mark(ST_L2P);
     ensure(far, n * sizeof(double4));
-    launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
+    if (!gen)
+      launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
+    else
+      launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, (double4*)far.p, s);
     mark(ST_P2P);
     ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
     launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
@@ -421,6 +452,30 @@ struct fmmh_ctx {
     p2p_interactions = read1((const unsigned long long*)p2p_work.p);
     HCHECK(hipGetLastError());
     evaluated = true;
+  }
+
+  // Rotation tables of the current p on the device (built once per p).
+  void rot_tables() {
+    const int p = prm.p;
+    if (rot_p == p) return;
+    std::vector<double> sw, swt, fact;
+    std::vector<cplx> din, dout, fin, fout;
+    rot_tables_host(p, sw, swt, din, dout, fin, fout, fact);
+    auto up = [&](Buf& b, const void* src, size_t bytes) {
+      ensure(b, bytes);
+      HCHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(rt_sw, sw.data(), sw.size() * 8);
+    up(rt_swt, swt.data(), swt.size() * 8);
+    up(rt_din, din.data(), din.size() * sizeof(cplx));
+    up(rt_dout, dout.data(), dout.size() * sizeof(cplx));
+    up(rt_fin, fin.data(), fin.size() * sizeof(cplx));
+    up(rt_fout, fout.data(), fout.size() * sizeof(cplx));
+    up(rt_fact, fact.data(), fact.size() * 8);
+    rot_struct.resize(rot_tables_struct_bytes());
+    make_rot_tables((double*)rt_sw.p, (double*)rt_swt.p, (cplx*)rt_din.p, (cplx*)rt_dout.p,
+                    (cplx*)rt_fin.p, (cplx*)rt_fout.p, (double*)rt_fact.p, rot_struct.data());
+    rot_p = p;
   }
 
   void walk(bool use_err) {

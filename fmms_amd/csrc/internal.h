@@ -129,6 +129,26 @@ bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s);
 
+// ------------------------------------------- generic_up.hip / generic_far.hip
+constexpr int TEMPLATED_MAX_P = 12;  // above this the runtime-p kernels run
+void launch_p2m_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                    cplx* moments, hipStream_t s);
+void launch_m2m_gen(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* moments,
+                    hipStream_t s);
+void rot_tables_host(int p, std::vector<double>& sw, std::vector<double>& swt,
+                     std::vector<cplx>& din, std::vector<cplx>& dout, std::vector<cplx>& fin,
+                     std::vector<cplx>& fout, std::vector<double>& fact);
+size_t rot_tables_struct_bytes();
+void make_rot_tables(const double* sw, const double* swt, const cplx* din, const cplx* dout,
+                     const cplx* fin, const cplx* fout, const double* fact, void* out);
+void launch_m2l_gen(int p, uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
+                    DevCells c, const cplx* moments, double soft2, const void* rot_tables,
+                    cplx* locals, hipStream_t s);
+void launch_l2l_gen(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
+                    hipStream_t s);
+void launch_l2p_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                    const cplx* locals, double4* far, hipStream_t s);
+
 // ---------------------------------------------------------- downward.hip
 bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
                 hipStream_t s);

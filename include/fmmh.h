@@ -54,7 +54,7 @@ extern "C" {
 #define FMMH_FLAG_TIMING 1u   /* record per-stage device times (hipEvents) */
 
 typedef struct fmmh_params {
-  int p;            /* expansion order (Fmm.p); supported 1..FMMH_MAX_P */
+  int p;            /* expansion order (Fmm.p); 1..FMMH_MAX_P (rotation M2L from p = 20) */
   double eps_mac;   /* error-controlled MAC tolerance (Fmm.eps, mac.rs:8-11,92);
                        +INFINITY = theta-only MAC, no f pre-pass */
   double theta_max; /* opening angle bound; 1.0 = the reference's hard-coded theta<1 */
@@ -66,7 +66,7 @@ typedef struct fmmh_params {
   double hbm_budget_bytes; /* device-memory budget of this context; 0 = no limit */
 } fmmh_params;
 
-#define FMMH_MAX_P 12
+#define FMMH_MAX_P 40
 
 typedef struct fmmh_ctx fmmh_ctx;
 
