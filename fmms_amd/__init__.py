@@ -93,6 +93,12 @@ SIGNATURES = {
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_plummer_trunc": (None, [_sz, _pd, ct.c_double, ct.c_uint64, _pd, _pd]),
+    "fmmh_ic_plummer": (ct.c_int, [_sz, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
+    "fmmh_ic_hernquist": (ct.c_int, [_sz, ct.c_double, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
+    "fmmh_ic_disk": (ct.c_int, [_sz, ct.c_double, ct.c_double, ct.c_double, ct.c_double,
+                                ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
+    "fmmh_ic_halos_cube": (ct.c_int, [_sz, ct.c_int, ct.c_double, ct.c_double, ct.c_double,
+                                      ct.c_double, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
 }
 
 
@@ -308,6 +314,55 @@ def hernquist(n, center=(0.0, 0.0, 0.0), seed=7):
     c = _f64(center)
     lib().fmmh_hernquist(n, _p(c), seed, _p(pos), _p(mass))
     return pos, mass
+
+
+# ------------------------------------------------- initial-condition library
+class System:
+    """Bodies of one initial condition: pos [n,3], vel [n,3], mass [n] (G = 1)."""
+
+    def __init__(self, pos, vel, mass):
+        self.pos, self.vel, self.mass = pos, vel, mass
+
+    @property
+    def n(self):
+        return self.mass.shape[0]
+
+
+def _ic(fn, n, *args):
+    if n <= 0:
+        raise FmmError(FMMH_E_PARAM, "n must be positive")
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    mass = np.empty(n)
+    rc = fn(n, *args, _p(pos), _p(vel), _p(mass))
+    if rc != FMMH_OK:
+        raise FmmError(rc, f"{fn.__name__}: invalid arguments")
+    return System(pos, vel, mass)
+
+
+def ic_plummer(n, rmax=20.0, seed=42):
+    """Plummer (a = 1, total mass 1) truncated at rmax with isotropic
+    distribution-function velocities; positions equal plummer_truncated's."""
+    return _ic(lib().fmmh_ic_plummer, n, float(rmax), seed)
+
+
+def ic_hernquist(n, a=1.0, rmax=100.0, seed=7):
+    """Hernquist (scale a, total mass 1) truncated at rmax, Jeans-Maxwellian
+    velocities below 0.95 v_esc."""
+    return _ic(lib().fmmh_ic_hernquist, n, float(a), float(rmax), seed)
+
+
+def ic_disk(n, rd=1.0, zd=0.05, rmax=10.0, m_central=0.0, sigma=0.0, seed=11):
+    """Exponential sech^2 disk on circular orbits (Keplerian + Freeman v_c)."""
+    return _ic(lib().fmmh_ic_disk, n, float(rd), float(zd), float(rmax), float(m_central),
+               float(sigma), seed)
+
+
+def ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
+                  rmax_over_a=20.0, seed=2024):
+    """Hernquist halos in a uniform unit-cube background (halo bodies first)."""
+    return _ic(lib().fmmh_ic_halos_cube, n, int(nhalo), float(halo_frac), float(a_min),
+               float(a_max), float(mass_decades), float(rmax_over_a), seed)
 
 
 def sample_ids(n, samp, seed=7):

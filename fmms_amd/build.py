@@ -44,6 +44,7 @@ SOURCES = {
     "generic_up.hip": "off",
     "generic_far.hip": "fast",
     "fmmh.hip": "off",
+    "ic.hip": "off",
 }
 
 

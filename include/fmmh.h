@@ -133,6 +133,37 @@ void fmmh_hernquist(size_t n, const double center[3], uint64_t seed, double* pos
 void fmmh_plummer_trunc(size_t n, const double center[3], double rmax, uint64_t seed, double* pos,
                         double* mass);
 
+/* Initial-condition library with velocities (host arrays pos/vel [n][3],
+ * mass [n]; G = 1, total mass 1; FMMH_OK or FMMH_E_PARAM). Extensions of
+ * models.rs:35-82 (positions drawn as there) that replace the sigma(r)
+ * Maxwellian of crates/fmm/tests/energy_gate.rs:39-79; no reference
+ * counterpart beyond those.
+ *   fmmh_ic_plummer    Plummer (a = 1) truncated at rmax, isotropic
+ *                      distribution-function velocities (Aarseth, Henon &
+ *                      Wielen 1974); positions equal fmmh_plummer_trunc's.
+ *   fmmh_ic_hernquist  Hernquist (scale a) truncated at rmax, Maxwellian of
+ *                      the isotropic Jeans dispersion, |v| < 0.95 v_esc.
+ *   fmmh_ic_disk       exponential disk (scale length rd, sech^2 height zd,
+ *                      R <= rmax) on circular orbits, v_c^2 = m_central/R +
+ *                      Freeman's thin-disk term, plus isotropic Gaussian
+ *                      dispersion sigma; m_central > 0 puts body 0 at rest at
+ *                      the origin with that mass.
+ *   fmmh_ic_halos_cube nhalo Hernquist halos carrying halo_frac of the mass
+ *                      (masses log-uniform over mass_decades, scale radii
+ *                      log-uniform in [a_min, a_max], centres uniform in the
+ *                      unit cube, truncated at rmax_over_a * a) and a cold
+ *                      uniform unit-cube background with the rest; equal
+ *                      body masses 1/n, halo bodies first.
+ * All but the halos have their mass-weighted mean velocity removed. */
+int fmmh_ic_plummer(size_t n, double rmax, uint64_t seed, double* pos, double* vel, double* mass);
+int fmmh_ic_hernquist(size_t n, double a, double rmax, uint64_t seed, double* pos, double* vel,
+                      double* mass);
+int fmmh_ic_disk(size_t n, double rd, double zd, double rmax, double m_central, double sigma,
+                 uint64_t seed, double* pos, double* vel, double* mass);
+int fmmh_ic_halos_cube(size_t n, int nhalo, double halo_frac, double a_min, double a_max,
+                       double mass_decades, double rmax_over_a, uint64_t seed, double* pos,
+                       double* vel, double* mass);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,24 @@ void orc_plummer(size_t n, const double center[3], uint64_t seed, double* pos, d
 void orc_hernquist(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
 double orc_rng_unit_first(uint64_t seed, size_t k, double* out); /* first k draws of Rng::unit */
 
+/* Initial-condition library (ic_oracle.cpp; extensions of models.rs:35-82,
+ * same arguments as the product's fmmh_ic_*; 0 = ok, 2 = bad argument). */
+int orc_ic_plummer(size_t n, double rmax, uint64_t seed, double* pos, double* vel, double* mass);
+int orc_ic_hernquist(size_t n, double a, double rmax, uint64_t seed, double* pos, double* vel,
+                     double* mass);
+int orc_ic_disk(size_t n, double rd, double zd, double rmax, double m_central, double sigma,
+                uint64_t seed, double* pos, double* vel, double* mass);
+int orc_ic_halos_cube(size_t n, int nhalo, double halo_frac, double a_min, double a_max,
+                      double mass_decades, double rmax_over_a, uint64_t seed, double* pos,
+                      double* vel, double* mass);
+double orc_hernquist_sigma2(double r, double a);
+/* crates/fmm/tests/energy_gate.rs:39-123: warm start, T, W, |sum m v| / sum m|v| */
+void orc_warm_start(size_t n, const double* pos, const double* mass, const double* pot,
+                    double* vel);
+double orc_kinetic(size_t n, const double* mass, const double* vel);
+double orc_potential_energy(size_t n, const double* mass, const double* pot);
+double orc_momentum_rel(size_t n, const double* mass, const double* vel);
+
 /* tree.rs:28-46 */
 uint64_t orc_dilate(uint64_t x);
 uint64_t orc_morton_key(double x, double y, double z, const double lo[3], double inv_side);

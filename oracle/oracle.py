@@ -75,6 +75,16 @@ def _load():
         "orc_get_locals": (None, [ct.c_void_p, _pd]),
         "orc_get_near": (None, [ct.c_void_p, _pd]),
         "orc_get_times": (None, [ct.c_void_p, _pd]),
+        "orc_ic_plummer": (ct.c_int, [_sz, _d, ct.c_uint64, _pd, _pd, _pd]),
+        "orc_ic_hernquist": (ct.c_int, [_sz, _d, _d, ct.c_uint64, _pd, _pd, _pd]),
+        "orc_ic_disk": (ct.c_int, [_sz, _d, _d, _d, _d, _d, ct.c_uint64, _pd, _pd, _pd]),
+        "orc_ic_halos_cube": (ct.c_int, [_sz, ct.c_int, _d, _d, _d, _d, _d, ct.c_uint64, _pd,
+                                         _pd, _pd]),
+        "orc_hernquist_sigma2": (_d, [_d, _d]),
+        "orc_warm_start": (None, [_sz, _pd, _pd, _pd, _pd]),
+        "orc_kinetic": (_d, [_sz, _pd, _pd]),
+        "orc_potential_energy": (_d, [_sz, _pd, _pd]),
+        "orc_momentum_rel": (_d, [_sz, _pd, _pd]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -124,6 +134,63 @@ def hernquist(n, center=(0.0, 0.0, 0.0), seed=7):
     c = _f64(center)
     lib().orc_hernquist(n, _p(c), seed, _p(pos), _p(mass))
     return pos, mass
+
+
+# ------------------------------------------- initial conditions (ic_oracle.cpp)
+def _ic(fn, n, *args):
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    mass = np.empty(n)
+    rc = fn(n, *args, _p(pos), _p(vel), _p(mass))
+    if rc != 0:
+        raise ValueError(f"{fn.__name__}: bad arguments")
+    return pos, vel, mass
+
+
+def ic_plummer(n, rmax=20.0, seed=42):
+    return _ic(lib().orc_ic_plummer, n, float(rmax), seed)
+
+
+def ic_hernquist(n, a=1.0, rmax=100.0, seed=7):
+    return _ic(lib().orc_ic_hernquist, n, float(a), float(rmax), seed)
+
+
+def ic_disk(n, rd=1.0, zd=0.05, rmax=10.0, m_central=0.0, sigma=0.0, seed=11):
+    return _ic(lib().orc_ic_disk, n, float(rd), float(zd), float(rmax), float(m_central),
+               float(sigma), seed)
+
+
+def ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
+                  rmax_over_a=20.0, seed=2024):
+    return _ic(lib().orc_ic_halos_cube, n, int(nhalo), float(halo_frac), float(a_min),
+               float(a_max), float(mass_decades), float(rmax_over_a), seed)
+
+
+def hernquist_sigma2(r, a=1.0):
+    return lib().orc_hernquist_sigma2(float(r), float(a))
+
+
+def warm_start(pos, mass, pot):
+    """energy_gate.rs:39-79: virialised sigma(r) Maxwellian velocities."""
+    pos, mass, pot = _f64(pos), _f64(mass), _f64(pot)
+    vel = np.empty_like(pos)
+    lib().orc_warm_start(mass.shape[0], _p(pos), _p(mass), _p(pot), _p(vel))
+    return vel
+
+
+def kinetic(mass, vel):
+    mass, vel = _f64(mass), _f64(vel)
+    return lib().orc_kinetic(mass.shape[0], _p(mass), _p(vel))
+
+
+def potential_energy(mass, pot):
+    mass, pot = _f64(mass), _f64(pot)
+    return lib().orc_potential_energy(mass.shape[0], _p(mass), _p(pot))
+
+
+def momentum_rel(mass, vel):
+    mass, vel = _f64(mass), _f64(vel)
+    return lib().orc_momentum_rel(mass.shape[0], _p(mass), _p(vel))
 
 
 def rng_units(seed, k):
