@@ -58,6 +58,15 @@ class _Params(ct.Structure):
                 ("precision", ct.c_int), ("flags", ct.c_uint), ("hbm_budget_bytes", ct.c_double)]
 
 
+class _LeapOpts(ct.Structure):
+    _fields_ = [("dt", ct.c_double), ("steps", ct.c_int), ("refresh_every", ct.c_int),
+                ("dt_eta", ct.c_double), ("dt_eps_len", ct.c_double), ("accel_cap", ct.c_double),
+                ("flags", ct.c_uint)]
+
+
+FMMH_LF_ACC_VALID = 1
+FMMH_LF_GRAPH = 2
+
 _pd = ct.POINTER(ct.c_double)
 _pu32 = ct.POINTER(ct.c_uint32)
 _pi32 = ct.POINTER(ct.c_int32)
@@ -93,6 +102,11 @@ SIGNATURES = {
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_plummer_trunc": (None, [_sz, _pd, ct.c_double, ct.c_uint64, _pd, _pd]),
+    "fmmh_evaluate_frozen": (ct.c_int, [_vp, _sz, _pd, _pd, _pd, _pd]),
+    "fmmh_evaluate_frozen_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
+    "fmmh_leapfrog_default": (None, [ct.POINTER(_LeapOpts)]),
+    "fmmh_leapfrog_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp,
+                                        ct.POINTER(_LeapOpts), _pd]),
     "fmmh_ic_plummer": (ct.c_int, [_sz, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
     "fmmh_ic_hernquist": (ct.c_int, [_sz, ct.c_double, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
     "fmmh_ic_disk": (ct.c_int, [_sz, ct.c_double, ct.c_double, ct.c_double, ct.c_double,
@@ -236,6 +250,39 @@ class Fmm:
         self._check(lib().fmmh_evaluate_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
                                               _vp(pot_ptr) if pot_ptr else None))
 
+    # ---- temporal reuse and time integration
+    def run_frozen(self, pos, mass):
+        """Evaluate on the tree and lists of the last full evaluation
+        (gpufmm.rs:2710-2741 frozen step): no rebuild, no walk."""
+        mass = _f64(mass)
+        n = mass.shape[0]
+        pos = _f64(pos, (n, 3))
+        acc = np.empty((n, 3))
+        pot = np.empty(n)
+        self._check(lib().fmmh_evaluate_frozen(self._h, n, _p(pos), _p(mass), _p(acc), _p(pot)))
+        return Result(acc, pot, self.n_cells)
+
+    def evaluate_frozen_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr):
+        self._check(lib().fmmh_evaluate_frozen_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr),
+                                                     _vp(acc_ptr), _vp(pot_ptr)))
+
+    def leapfrog_device(self, n, pos_ptr, vel_ptr, mass_ptr, acc_ptr, pot_ptr, dt, steps,
+                        refresh_every=1, dt_eta=0.0, dt_eps_len=0.02, accel_cap=0.0,
+                        acc_valid=False, graph=False):
+        """Kick-drift-kick steps on device-resident id-order state
+        (sim.rs:539-594); returns the time advanced."""
+        o = _LeapOpts()
+        lib().fmmh_leapfrog_default(ct.byref(o))
+        o.dt, o.steps, o.refresh_every = float(dt), int(steps), int(refresh_every)
+        o.dt_eta, o.dt_eps_len, o.accel_cap = float(dt_eta), float(dt_eps_len), float(accel_cap)
+        o.flags = (FMMH_LF_ACC_VALID if acc_valid else 0) | (FMMH_LF_GRAPH if graph else 0)
+        t = ct.c_double(0.0)
+        self._check(lib().fmmh_leapfrog_device(self._h, n, _vp(pos_ptr), _vp(vel_ptr),
+                                              _vp(mass_ptr), _vp(acc_ptr), _vp(pot_ptr),
+                                              ct.byref(o), ct.byref(t)))
+        self.n = n
+        return t.value
+
     # ---- parity views
     def tree(self):
         n, nc = self.n, self.n_cells
@@ -363,6 +410,41 @@ def ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decade
     """Hernquist halos in a uniform unit-cube background (halo bodies first)."""
     return _ic(lib().fmmh_ic_halos_cube, n, int(nhalo), float(halo_frac), float(a_min),
                float(a_max), float(mass_decades), float(rmax_over_a), seed)
+
+
+class Integrator:
+    """Device-resident N-body state advanced by kick-drift-kick leapfrog with
+    this library's FMM forces (the energy-gate loop of
+    crates/fmm/tests/energy_gate.rs:156-177 and Sim::step, sim.rs:539-594).
+    pos/vel [n,3], mass [n] are copied to the device once; `step` runs on the
+    device; `state()` copies pos, vel, acc, pot back."""
+
+    def __init__(self, fmm, pos, vel, mass):
+        from . import hip
+
+        mass = _f64(mass)
+        n = mass.shape[0]
+        self.fmm, self.n = fmm, n
+        self.pos = hip.DeviceArray.from_host(_f64(pos, (n, 3)))
+        self.vel = hip.DeviceArray.from_host(_f64(vel, (n, 3)))
+        self.mass = hip.DeviceArray.from_host(mass)
+        self.acc = hip.DeviceArray((n, 3))
+        self.pot = hip.DeviceArray((n,))
+        self.acc_valid = False
+        self.time = 0.0
+
+    def step(self, dt, steps=1, refresh_every=1, graph=False, **kw):
+        t = self.fmm.leapfrog_device(self.n, self.pos.ptr, self.vel.ptr, self.mass.ptr,
+                                     self.acc.ptr, self.pot.ptr, dt, steps,
+                                     refresh_every=refresh_every, acc_valid=self.acc_valid,
+                                     graph=graph, **kw)
+        self.acc_valid = True
+        self.time += t
+        return t
+
+    def state(self):
+        return (self.pos.download(), self.vel.download(), self.acc.download(),
+                self.pot.download())
 
 
 def sample_ids(n, samp, seed=7):

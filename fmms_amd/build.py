@@ -45,6 +45,7 @@ SOURCES = {
     "generic_far.hip": "fast",
     "fmmh.hip": "off",
     "ic.hip": "off",
+    "integrate.hip": "off",
 }
 
 

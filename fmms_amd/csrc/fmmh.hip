@@ -112,11 +112,12 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
-                  &rt_fact, &direct_ids, &direct_out};
+                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state};
     bufs.assign(std::begin(all), std::end(all));
   }
 
   ~fmmh_ctx() {
+    drop_graph();
     for (Buf* b : bufs)
       if (b->p) (void)hipFree(b->p);
     if (pinned.p) (void)hipHostFree(pinned.p);
@@ -169,7 +170,7 @@ struct fmmh_ctx {
   }
 
   void mark(int st) {
-    if (!(prm.flags & FMMH_FLAG_TIMING)) return;
+    if (!(prm.flags & FMMH_FLAG_TIMING) || capturing) return;
     if (!ev_init) {
       for (auto& e : ev) HCHECK(hipEventCreate(&e));
       ev_init = true;
@@ -218,7 +219,9 @@ struct fmmh_ctx {
   // ------------------------------------------------------------------ build
   void build(size_t n_, const double* pos, const double* mass) {
     n = n_;
-    built = evaluated = false;
+    built = evaluated = lists_ok = false;
+    tree_evals = 0;
+    drop_graph();
     const int p = prm.p;
     (void)p;
     ensure(keys, n * 8);
@@ -320,61 +323,67 @@ struct fmmh_ctx {
   }
 
   // --------------------------------------------------------------- evaluate
-  void evaluate(const double* pos, const double* mass, double* acc, double* pot, bool regather) {
+  // Lists of the last full evaluation, kept for frozen evaluations until the
+  // next build (directed M2L / P2P keys sorted by target, CSR, M2L chunks).
+  bool lists_ok = false;
+  const uint64_t* msorted = nullptr;
+  const uint64_t* psorted = nullptr;
+  uint32_t pnnz = 0, nchunks = 0;
+  uint32_t tree_evals = 0;  // evaluations run on the current tree and lists
+  bool capturing = false;
+
+  // Gather (optional) + P2M + M2M into `moments` (already sized).
+  void upward(const double* pos, const double* mass, bool regather) {
     const int p = prm.p;
     const size_t D = (size_t)hdim(p);
-    rot_struct.resize(std::max(rot_struct.size(), rot_tables_struct_bytes()));
     DevCells c = cells();
     DevBodies b = bodies();
-    if (regather) {
-      HCHECK(hipMemsetAsync(flags.p, 0, 64, s));
-      launch_gather(pos, mass, order, n, b, s);
-    }
+    if (regather) launch_gather(pos, mass, order, n, b, s);
     mark(ST_UPWARD);
-    cplx* mom = ensure<cplx>(moments, nc * D * sizeof(cplx));
-    cplx* loc = ensure<cplx>(locals, nc * D * sizeof(cplx));
-    const bool gen = p > TEMPLATED_MAX_P;
-    if (!gen) {
+    cplx* mom = (cplx*)moments.p;
+    if (p <= TEMPLATED_MAX_P) {
       launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
         launch_m2m(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom, s);
     } else {
-      HCHECK(hipMemsetAsync(mom, 0, nc * D * sizeof(cplx), s));
+      launch_fill_u64(mom, nc * D * 2, 0, s);  // kernel fill: safe under graph capture
       launch_p2m_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
         launch_m2m_gen(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom,
                        s);
     }
-    mark(ST_PREPASS);
-    const bool use_err = std::isfinite(prm.eps_mac);
-    {
-      std::vector<double> f(2 * p + 1, 1.0);
-      for (size_t i = 1; i < f.size(); i++) f[i] = f[i - 1] * (double)i;  // moments.rs:11-17
-      std::vector<double> bn(p + 1, 1.0);
-      for (int k = 0; k <= p; k++) bn[k] = f[p] / (f[k] * f[p - k]);  // moments.rs:20-27
-      double* hf = (double*)host_stage((f.size() + bn.size()) * 8);
-      std::memcpy(hf, f.data(), f.size() * 8);
-      std::memcpy(hf + f.size(), bn.data(), bn.size() * 8);
-      ensure(fact, f.size() * 8);
-      ensure(binom, bn.size() * 8);
-      HCHECK(hipMemcpyAsync(fact.p, hf, f.size() * 8, hipMemcpyHostToDevice, s));
-      HCHECK(hipMemcpyAsync(binom.p, hf + f.size(), bn.size() * 8, hipMemcpyHostToDevice, s));
-      HCHECK(hipStreamSynchronize(s));
-    }
+  }
+
+  // Multipole powers, f pre-pass and fmin (error-controlled MAC only).
+  void mac_inputs() {
+    const int p = prm.p;
+    DevCells c = cells();
+    DevBodies b = bodies();
+    std::vector<double> f(2 * p + 1, 1.0);
+    for (size_t i = 1; i < f.size(); i++) f[i] = f[i - 1] * (double)i;  // moments.rs:11-17
+    std::vector<double> bn(p + 1, 1.0);
+    for (int k = 0; k <= p; k++) bn[k] = f[p] / (f[k] * f[p - k]);  // moments.rs:20-27
+    double* hf = (double*)host_stage((f.size() + bn.size()) * 8);
+    std::memcpy(hf, f.data(), f.size() * 8);
+    std::memcpy(hf + f.size(), bn.data(), bn.size() * 8);
+    ensure(fact, f.size() * 8);
+    ensure(binom, bn.size() * 8);
+    HCHECK(hipMemcpyAsync(fact.p, hf, f.size() * 8, hipMemcpyHostToDevice, s));
+    HCHECK(hipMemcpyAsync(binom.p, hf + f.size(), bn.size() * 8, hipMemcpyHostToDevice, s));
+    HCHECK(hipStreamSynchronize(s));
     ensure(fmin, nc * 8);
-    if (use_err) {
-      ensure(pows, (size_t)nc * (p + 1) * 8);
-      ensure(fvals, n * 8);
-      launch_pows(p, nc, mom, (double*)fact.p, (double*)pows.p, s);
-      launch_fprepass((uint32_t)n, nc, c, b, (double*)fvals.p, s);
-      for (int L = (int)max_level; L >= 0; L--)
-        launch_fmin((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c,
-                    (double*)fvals.p, (double*)fmin.p, s);
-    }
-    mark(ST_WALK);
-    walk(use_err);
-    mark(ST_LISTS);
-    // directed M2L entries grouped by target
+    ensure(pows, (size_t)nc * (p + 1) * 8);
+    ensure(fvals, n * 8);
+    launch_pows(p, nc, (const cplx*)moments.p, (double*)fact.p, (double*)pows.p, s);
+    launch_fprepass((uint32_t)n, nc, c, b, (double*)fvals.p, s);
+    for (int L = (int)max_level; L >= 0; L--)
+      launch_fmin((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c,
+                  (double*)fvals.p, (double*)fmin.p, s);
+  }
+
+  // Directed lists grouped by target, P2P work count, M2L chunks.
+  void build_lists() {
+    const int p = prm.p;
     const int kbits = 2 * sbits;
     ensure(mkeys, (size_t)2 * n_m2l * 8);
     ensure(mkeys_alt, (size_t)2 * n_m2l * 8);
@@ -383,9 +392,8 @@ struct fmmh_ctx {
     launch_directed_keys_m2l((uint2*)m2l.p, n_m2l, sbits, (uint64_t*)mkeys.p, s);
     int fl = radix_sort_u64((uint64_t*)mkeys.p, nullptr, (uint64_t*)mkeys_alt.p, nullptr,
                             (size_t)2 * n_m2l, 0, kbits, (uint32_t*)sort_tmp.p, s);
-    const uint64_t* msorted = (const uint64_t*)(fl ? mkeys_alt.p : mkeys.p);
+    msorted = (const uint64_t*)(fl ? mkeys_alt.p : mkeys.p);
     launch_csr_ptr(msorted, 2 * n_m2l, sbits, nc, (uint32_t*)tptr.p, s);
-    // directed P2P entries grouped by target
     ensure(pkeys, (size_t)2 * n_p2p * 8);
     ensure(pkeys_alt, (size_t)2 * n_p2p * 8);
     ensure(pptr, ((size_t)nc + 1) * 4);
@@ -395,15 +403,14 @@ struct fmmh_ctx {
     launch_directed_keys_p2p((uint2*)p2p.p, n_p2p, sbits, (uint64_t*)pkeys.p, (uint32_t*)pcount.p, s);
     fl = radix_sort_u64((uint64_t*)pkeys.p, nullptr, (uint64_t*)pkeys_alt.p, nullptr,
                         (size_t)2 * n_p2p, 0, kbits, (uint32_t*)sort_tmp.p, s);
-    const uint64_t* psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
+    psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
     uint32_t nself = read1((const uint32_t*)pcount.p);
-    uint32_t pnnz = 2 * n_p2p - nself;
+    pnnz = 2 * n_p2p - nself;
     launch_csr_ptr(psorted, pnnz, sbits, nc, (uint32_t*)pptr.p, s);
     ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
     launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
                      (unsigned long long*)p2p_work.p, s);
-    if (!gen) {
-      // M2L chunks
+    if (p <= TEMPLATED_MAX_P) {
       size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
       ensure(nch, ((size_t)nc + 1) * 4);
       ensure(choff, ((size_t)nc + 1) * 4);
@@ -414,18 +421,33 @@ struct fmmh_ctx {
                        (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
       // choff[nc] = total chunks (the reduce reads choff[t+1])
       HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
-      uint32_t nchunks = read1(tot);
+      nchunks = read1(tot);
       ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
-      mark(ST_M2L);
-      launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, prm.softening * prm.softening,
-                        (cplx*)part.p, s);
+    } else if (p >= 20) {
+      rot_tables();
+    }
+    ensure(far, n * sizeof(double4));
+    ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
+  }
+
+  // M2L (+ reduce), L2L, L2P and the fused P2P + scatter into acc / pot.
+  // Launches only: safe to capture into a graph once the lists exist.
+  void far_near(double* acc, double* pot) {
+    const int p = prm.p;
+    DevCells c = cells();
+    DevBodies b = bodies();
+    const cplx* mom = (const cplx*)moments.p;
+    cplx* loc = (cplx*)locals.p;
+    const double soft2 = prm.softening * prm.softening;
+    const bool gen = p > TEMPLATED_MAX_P;
+    mark(ST_M2L);
+    if (!gen) {
+      launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, soft2, (cplx*)part.p, s);
       mark(ST_M2L_REDUCE);
       launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
     } else {
-      if (p >= 20) rot_tables();
-      mark(ST_M2L);
-      launch_m2l_gen(p, nc, (uint32_t*)tptr.p, msorted, sbits, c, mom,
-                     prm.softening * prm.softening, rot_struct.data(), loc, s);
+      launch_m2l_gen(p, nc, (uint32_t*)tptr.p, msorted, sbits, c, mom, soft2, rot_struct.data(),
+                     loc, s);
       mark(ST_M2L_REDUCE);
     }
     mark(ST_L2L);
@@ -438,20 +460,135 @@ struct fmmh_ctx {
         launch_l2l_gen(p, lc, cnt, c, loc, s);
     }
     mark(ST_L2P);
-    ensure(far, n * sizeof(double4));
     if (!gen)
       launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, (double4*)far.p, s);
     mark(ST_P2P);
-    ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
     launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
-               b.xyzm, (uint2*)prange.p, prm.softening * prm.softening, (double4*)far.p, order,
-               acc, pot, s);
+               b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
     mark(ST_COUNT);
+  }
+
+  void evaluate(const double* pos, const double* mass, double* acc, double* pot, bool regather) {
+    const size_t D = (size_t)hdim(prm.p);
+    rot_struct.resize(std::max(rot_struct.size(), rot_tables_struct_bytes()));
+    lists_ok = false;
+    drop_graph();
+    ensure(moments, nc * D * sizeof(cplx));
+    ensure(locals, nc * D * sizeof(cplx));
+    upward(pos, mass, regather);
+    mark(ST_PREPASS);
+    const bool use_err = std::isfinite(prm.eps_mac);
+    if (use_err) mac_inputs();
+    else ensure(fmin, nc * 8);
+    mark(ST_WALK);
+    walk(use_err);
+    mark(ST_LISTS);
+    build_lists();
+    far_near(acc, pot);
     p2p_interactions = read1((const unsigned long long*)p2p_work.p);
     HCHECK(hipGetLastError());
     evaluated = true;
+    lists_ok = true;
+    tree_evals = 1;
+  }
+
+  // Temporal reuse (gpufmm.rs:2710-2741 frozen_step): bodies re-gathered
+  // into the built tree's slots, expansions recomputed about the frozen cell
+  // centres, interactions from the last full evaluation's lists.
+  void evaluate_frozen(const double* pos, const double* mass, double* acc, double* pot) {
+    upward(pos, mass, true);
+    mark(ST_PREPASS);
+    mark(ST_WALK);
+    mark(ST_LISTS);
+    far_near(acc, pot);
+    tree_evals++;
+  }
+
+  // ----------------------------------------------------------- leapfrog
+  Buf lf_part, lf_state;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    const void *pos, *vel, *mass, *acc, *pot;
+    double dt, eta, eps_len, cap;
+    bool operator==(const GraphKey& o) const {
+      return pos == o.pos && vel == o.vel && mass == o.mass && acc == o.acc && pot == o.pot &&
+             dt == o.dt && eta == o.eta && eps_len == o.eps_len && cap == o.cap;
+    }
+  } gkey{};
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  // One step on the current tree and lists: dt, kick + drift, frozen
+  // evaluation, optional cap, kick (sim.rs:548-572).
+  void frozen_step(double* pos, double* vel, const double* mass, double* acc, double* pot,
+                   const fmmh_leapfrog_opts& o, double* part, double* st) {
+    launch_step_dt(acc, n, o.dt, o.dt_eta, o.dt_eps_len, part, st, s);
+    launch_kick_drift(n, pos, vel, acc, st, s);
+    evaluate_frozen(pos, mass, acc, pot);
+    if (o.accel_cap > 0.0) launch_cap_accel(n, acc, o.accel_cap, s);
+    launch_kick(n, vel, acc, st, s);
+  }
+
+  void capture_frozen_step(double* pos, double* vel, const double* mass, double* acc, double* pot,
+                           const fmmh_leapfrog_opts& o, double* part, double* st) {
+    drop_graph();
+    const uint32_t age = tree_evals;  // capture records the step, it does not run it
+    capturing = true;
+    HCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    frozen_step(pos, vel, mass, acc, pot, o, part, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    capturing = false;
+    tree_evals = age;
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      throw HipError{std::string("hipStreamEndCapture: ") + hipGetErrorString(e)};
+    }
+    graph = g;
+    HCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    gkey = GraphKey{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap};
+  }
+
+  double leapfrog(size_t n_, double* pos, double* vel, const double* mass, double* acc,
+                  double* pot, const fmmh_leapfrog_opts& o) {
+    double* part = ensure<double>(lf_part, dt_scratch_words() * 8);
+    double* st = ensure<double>(lf_state, 64);
+    launch_fill_u64(st, 2, 0, s);
+    auto full = [&] {
+      build(n_, pos, mass);
+      evaluate(pos, mass, acc, pot, false);
+      if (o.accel_cap > 0.0) launch_cap_accel(n, acc, o.accel_cap, s);
+    };
+    if (!(o.flags & FMMH_LF_ACC_VALID) || !lists_ok || n != n_) full();
+    const bool use_graph = (o.flags & FMMH_LF_GRAPH) != 0;
+    const GraphKey key{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap};
+    for (int k = 0; k < o.steps; k++) {
+      if (tree_evals >= (uint32_t)o.refresh_every) {
+        launch_step_dt(acc, n, o.dt, o.dt_eta, o.dt_eps_len, part, st, s);
+        launch_kick_drift(n, pos, vel, acc, st, s);
+        full();
+        launch_kick(n, vel, acc, st, s);
+      } else if (use_graph) {
+        if (!gexec || !(gkey == key)) capture_frozen_step(pos, vel, mass, acc, pot, o, part, st);
+        HCHECK(hipGraphLaunch(gexec, s));
+        tree_evals++;
+      } else {
+        frozen_step(pos, vel, mass, acc, pot, o, part, st);
+      }
+    }
+    double* h = (double*)host_stage(16);
+    HCHECK(hipMemcpyAsync(h, st, 16, hipMemcpyDeviceToHost, s));
+    HCHECK(hipStreamSynchronize(s));
+    HCHECK(hipGetLastError());
+    return h[1];
   }
 
   // Rotation tables of the current p on the device (built once per p).
@@ -673,6 +810,50 @@ int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double
   });
 }
 
+int fmmh_evaluate_frozen_device(fmmh_ctx* ctx, size_t n, const double* d_pos,
+                                const double* d_mass, double* d_accel, double* d_pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!ctx->lists_ok) return fail(ctx, FMMH_E_STATE, "frozen evaluation needs a full evaluation");
+  if (n != ctx->n) return fail(ctx, FMMH_E_STATE, "n differs from the built tree");
+  return guarded(ctx, [&] {
+    ctx->evaluate_frozen(d_pos, d_mass, d_accel, d_pot);
+    HCHECK(hipGetLastError());
+    ctx->stage_ms[ST_TREE] = 0.0;
+    ctx->finish_timing(ST_UPWARD, ST_COUNT);
+    if (!(ctx->prm.flags & FMMH_FLAG_TIMING)) HCHECK(hipStreamSynchronize(ctx->s));
+  });
+}
+
+void fmmh_leapfrog_default(fmmh_leapfrog_opts* o) {
+  o->dt = 1e-4;
+  o->steps = 1;
+  o->refresh_every = 1;
+  o->dt_eta = 0.0;
+  o->dt_eps_len = 0.02;
+  o->accel_cap = 0.0;
+  o->flags = 0;
+}
+
+int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
+                         const double* d_mass, double* d_accel, double* d_pot,
+                         const fmmh_leapfrog_opts* o, double* time_advanced) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!o || !d_pos || !d_vel || !d_mass || !d_accel || !d_pot)
+    return fail(ctx, FMMH_E_PARAM, "null argument");
+  if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
+  if (!(o->dt > 0.0) || !std::isfinite(o->dt) || o->steps < 0 || o->refresh_every < 1 ||
+      !(o->dt_eta >= 0.0) || !(o->dt_eps_len > 0.0) || !(o->accel_cap >= 0.0) ||
+      (o->flags & ~(FMMH_LF_ACC_VALID | FMMH_LF_GRAPH)))
+    return fail(ctx, FMMH_E_PARAM,
+                "leapfrog options: dt > 0, steps >= 0, refresh_every >= 1, dt_eta >= 0, "
+                "dt_eps_len > 0, accel_cap >= 0, known flags only");
+  return guarded(ctx, [&] {
+    const double t = ctx->leapfrog(n, d_pos, d_vel, d_mass, d_accel, d_pot, *o);
+    if (time_advanced) *time_advanced = t;
+  });
+}
+
 static int upload(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass) {
   return guarded(ctx, [&] {
     ctx->ensure(ctx->in_pos, n * 24);
@@ -728,6 +909,21 @@ int fmmh_accel(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass, d
   if (!rc)
     rc = fmmh_accel_device(ctx, n, (const double*)ctx->in_pos.p, (const double*)ctx->in_mass.p,
                            (double*)ctx->out_acc.p, (double*)ctx->out_pot.p);
+  if (!rc) rc = download(ctx, n, accel, pot);
+  return rc;
+}
+
+int fmmh_evaluate_frozen(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass,
+                         double* accel, double* pot) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!ctx->lists_ok) return fail(ctx, FMMH_E_STATE, "frozen evaluation needs a full evaluation");
+  if (n != ctx->n) return fail(ctx, FMMH_E_STATE, "n differs from the built tree");
+  int rc = upload(ctx, n, pos, mass);
+  if (!rc) rc = outputs(ctx, n);
+  if (!rc)
+    rc = fmmh_evaluate_frozen_device(ctx, n, (const double*)ctx->in_pos.p,
+                                     (const double*)ctx->in_mass.p, (double*)ctx->out_acc.p,
+                                     (double*)ctx->out_pot.p);
   if (!rc) rc = download(ctx, n, accel, pot);
   return rc;
 }

@@ -166,6 +166,17 @@ size_t p2p_count_scratch(uint32_t np);
 void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
                       unsigned long long* out, hipStream_t s);
 
+// --------------------------------------------------------- integrate.hip
+// st[0] = dt of the current step, st[1] = time advanced (device scalars).
+size_t dt_scratch_words();
+void launch_step_dt(const double* acc, size_t n, double dt_req, double eta, double eps_len,
+                    double* part, double* st, hipStream_t s);
+void launch_kick_drift(size_t n, double* pos, double* vel, const double* acc, const double* st,
+                       hipStream_t s);
+void launch_kick(size_t n, double* vel, const double* acc, const double* st, hipStream_t s);
+void launch_cap_accel(size_t n, double* acc, double cap, hipStream_t s);
+void launch_fill_u64(void* p, size_t words, uint64_t v, hipStream_t s);
+
 // ------------------------------------------------------------ direct.hip
 void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,
                            const uint32_t* ids, uint32_t nt, double* out /*[nt][5]*/,

@@ -95,6 +95,48 @@ int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double
 /* Tree of the last build (tree.rs:49-70), DFS-preorder cell ids. Any
  * output pointer may be NULL. children [nc][8], -1 where absent; cen [nc][3]. */
 size_t fmmh_num_cells(const fmmh_ctx* ctx);
+
+/* Temporal reuse — the frozen step of crates/visual/src/gpufmm.rs:2710-2741
+ * (refresh model gpufmm.rs:9-20): evaluate on the tree and interaction lists
+ * of this context's last full evaluation. Bodies are re-gathered into the
+ * same slots, moments and locals are recomputed about the frozen cell
+ * centres, no tree build and no walk run. Needs a previous fmmh_evaluate* or
+ * fmmh_accel* with the same n (FMMH_E_STATE otherwise). The reference keeps
+ * a tree for at most k <= 8 steps. */
+int fmmh_evaluate_frozen(fmmh_ctx* ctx, size_t n, const double* pos, const double* mass,
+                         double* accel, double* pot);
+int fmmh_evaluate_frozen_device(fmmh_ctx* ctx, size_t n, const double* d_pos,
+                                const double* d_mass, double* d_accel, double* d_pot);
+
+/* Kick-drift-kick leapfrog on device-resident state (crates/visual/src/sim.rs:539-594;
+ * the energy-gate loop of crates/fmm/tests/energy_gate.rs:156-177): per step
+ *   dt = min(dt, dt_eta * sqrt(dt_eps_len / max|a|)) when dt_eta > 0 (sim.rs:19-20,553-557),
+ *   v += a dt/2, x += v dt, a = accel(x), [cap |a| (sim.rs:14-33) when accel_cap > 0],
+ *   v += a dt/2.
+ * The tree and lists are rebuilt when they have served refresh_every
+ * evaluations (1 = every step, as the energy gate does); steps in between are
+ * frozen evaluations. With FMMH_LF_GRAPH the frozen step (dt, kicks, drift and
+ * the whole force evaluation) is captured once per tree into a hipGraph and
+ * replayed. FMMH_LF_ACC_VALID: accel/pot already hold a(x) of the entry
+ * positions from this context's last evaluation (else they are evaluated
+ * first). pos/vel/accel [n][3], pot [n], mass [n] are device pointers in id
+ * order; on return accel/pot belong to the final positions and
+ * *time_advanced (host, may be NULL) is the sum of the steps' dt. */
+#define FMMH_LF_ACC_VALID 1u
+#define FMMH_LF_GRAPH 2u
+typedef struct fmmh_leapfrog_opts {
+  double dt;
+  int steps;
+  int refresh_every;
+  double dt_eta;     /* 0: fixed dt; sim.rs uses 0.25 */
+  double dt_eps_len; /* sim.rs uses 0.02 */
+  double accel_cap;  /* 0: off; sim.rs uses 50 */
+  unsigned flags;
+} fmmh_leapfrog_opts;
+void fmmh_leapfrog_default(fmmh_leapfrog_opts* out);
+int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
+                         const double* d_mass, double* d_accel, double* d_pot,
+                         const fmmh_leapfrog_opts* opts, double* time_advanced);
 int fmmh_get_tree(fmmh_ctx* ctx, uint32_t* order, uint64_t* keys_sorted, uint32_t* first,
                   uint32_t* count, int32_t* children, uint8_t* leaf, uint32_t* level,
                   double* cen, double* mass, double* rho);
