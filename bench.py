@@ -14,6 +14,14 @@ percentiles, relative force error against the FP64 direct sum on sampled
 targets, the roofline of the dominant kernel (HIP events on the library's
 stream over the timed region) and the CPU baseline (the oracle's threaded
 restatement of Fmm::run_threaded on this host).
+
+--workload halos: 16,777,216 bodies, 90 % of the mass in 64 Hernquist halos
+(masses log-uniform over two decades, scale radii 0.01-0.1, truncated at 20
+scale radii) in a uniform unit-cube background, softening 1e-4, p = 8,
+theta = 0.5; a step is one kick-drift-kick leapfrog step at dt = 1e-4 on
+device-resident state. The tree and lists are rebuilt every --refresh steps;
+the steps in between replay one captured hipGraph holding the whole step
+(dt, kicks, drift, upward pass, M2L, L2L, L2P, P2P).
 """
 import argparse
 import json
@@ -101,6 +109,128 @@ def cpu_baseline(pos, mass, args, n_leaf):
     }
 
 
+def stage_rooflines(stages_ms, wc, p):
+    """Achieved FP64 rates of the VALU stages (algorithmic flops / stage time)."""
+    work = {"m2l": 8.0 * t_full(p) * 2 * wc["m2l_pairs"], "p2p": 22.0 * wc["p2p_interactions"]}
+    out = {}
+    for k, w in work.items():
+        if stages_ms.get(k, 0) > 0:
+            ach = w / (stages_ms[k] * 1e-3) / 1e12
+            out[k] = {"ms": stages_ms[k], "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                      "frac": ach / FP64_PEAK_TFLOPS}
+    return out
+
+
+def bench_halos(fa, hip, args):
+    n = args.n if args.n != (1 << 20) else 1 << 24
+    soft = 1e-4
+    log(f"[bench] generating {n} bodies: 64 Hernquist halos in a unit cube")
+    t0 = time.perf_counter()
+    s = fa.ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
+                         rmax_over_a=20.0, seed=2024)
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
+    d_mass = hip.DeviceArray.from_host(s.mass)
+    d_pos0 = hip.DeviceArray.from_host(s.pos)
+    d_vel0 = hip.DeviceArray.from_host(s.vel)
+    d_pos = hip.DeviceArray((n, 3))
+    d_vel = hip.DeviceArray((n, 3))
+    d_acc = hip.DeviceArray((n, 3))
+    d_pot = hip.DeviceArray((n,))
+    n_leaf = args.n_leaf or 64
+    mk = lambda timing=False: fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf,
+                                      theta_max=args.theta, softening=soft, timing=timing)
+
+    def trajectory(ctx, steps):
+        d_pos.copy_from(d_pos0)
+        d_vel.copy_from(d_vel0)
+        # a(x0) first: the leapfrog's kicks need it (untimed, like the warm-up)
+        ctx.leapfrog_device(n, d_pos.ptr, d_vel.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr, args.dt, 0)
+        hip.synchronize()
+        t = time.perf_counter()
+        ctx.leapfrog_device(n, d_pos.ptr, d_vel.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr, args.dt,
+                            steps, refresh_every=args.refresh, acc_valid=True, graph=True)
+        hip.synchronize()
+        return time.perf_counter() - t
+
+    ctx = mk()
+    log("[bench] warm-up trajectory")
+    trajectory(ctx, max(1, args.warmup))
+    log(f"[bench] timing {args.steps} KDK steps (refresh every {args.refresh})")
+    st0 = ctx.leapfrog_stats()
+    elapsed = trajectory(ctx, args.steps)
+    st1 = ctx.leapfrog_stats()
+    lf = {k: st1[k] - st0[k] for k in st1}
+    lf["rebuilds"] -= 1  # the untimed evaluation of a(x0)
+    log(f"[bench] leapfrog: {lf}")
+    wc = ctx.work_counts()
+    acc = d_acc.download()
+    pos = d_pos.download()
+    ctx.close()
+    # per-stage times: one full and one frozen evaluation with stage events
+    tctx = mk(timing=True)
+    for _ in range(2):  # the first call grows the context's buffers
+        tctx.accel_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
+    full_ms = tctx.stage_times()
+    acc_fresh = d_acc.download()
+    tctx.evaluate_frozen_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
+    frozen_ms = tctx.stage_times()
+    tctx.close()
+    roof_stages = stage_rooflines(frozen_ms, wc, args.p)
+    dom = max(roof_stages, key=lambda k: roof_stages[k]["ms"])
+    roof = {"kernel": dom, "bound": "valu", "achieved": round(roof_stages[dom]["achieved"], 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(roof_stages[dom]["frac"], 4),
+            "traffic": None, "avg_launch_ms": round(roof_stages[dom]["ms"], 4)}
+    err = None
+    if not args.no_error:
+        ids = fa.sample_ids(n, args.samples, 7)
+        log(f"[bench] FP64 direct sum on {len(ids)} sampled targets")
+        a_ref, _, f_ref = fa.direct_sampled(pos, s.mass, ids, softening=soft)
+        nh = int(math.floor(0.9 * n + 0.5))  # halo bodies come first
+
+        def stats(a):
+            rel = fa.rel_errors(a[ids], a_ref)
+            scaled = np.linalg.norm(a[ids] - a_ref, axis=1) / f_ref
+            halo = ids < nh
+            return {"rel_median": float(np.median(rel)), "rel_p99": float(np.percentile(rel, 99)),
+                    "rel_max": float(rel.max()),
+                    "rel_p99_halo_bodies": float(np.percentile(rel[halo], 99)),
+                    "rel_p99_background": float(np.percentile(rel[~halo], 99)),
+                    "scaled_rms": float(np.sqrt(np.mean(scaled ** 2))),
+                    "scaled_max": float(scaled.max())}
+
+        err = {"targets": int(len(ids)), **stats(acc),
+               "fresh_tree": stats(acc_fresh)}
+        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}; "
+            f"fresh tree p99 {err['fresh_tree']['rel_p99']:.3e}")
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / elapsed,
+        "unit": "body-forces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Hernquist halos in a uniform cube)",
+        "config": {"workload": f"halos64_cube_N{n}_p{args.p}_theta{args.theta}_soft{soft}_kdk",
+                   "n": n, "p": args.p, "theta": args.theta, "softening": soft,
+                   "n_leaf": n_leaf, "dt": args.dt, "refresh_every": args.refresh,
+                   "drift_tol": 0.25, "graph": True, "precision": "fp64"},
+        "leapfrog": lf,
+        "error": err,
+        "roofline": roof,
+        "stages_ms_full": {k: round(v, 4) for k, v in full_ms.items()},
+        "stages_ms_frozen": {k: round(v, 4) for k, v in frozen_ms.items()},
+        "stage_roofline": roof_stages,
+        "work": wc,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +244,10 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--workload", choices=["plummer", "halos"], default="plummer")
+    ap.add_argument("--refresh", type=int, default=10,
+                    help="halos: rebuild the tree and lists every this many steps")
+    ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device (a copy kernel of known "
                          "size, for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -128,6 +262,9 @@ def main():
 
     import fmms_amd as fa
     from fmms_amd import hip
+
+    if args.workload == "halos":
+        return bench_halos(fa, hip, args)
 
     n = args.n
     log(f"[bench] generating Plummer N={n} (r <= 20)")

@@ -61,7 +61,7 @@ class _Params(ct.Structure):
 class _LeapOpts(ct.Structure):
     _fields_ = [("dt", ct.c_double), ("steps", ct.c_int), ("refresh_every", ct.c_int),
                 ("dt_eta", ct.c_double), ("dt_eps_len", ct.c_double), ("accel_cap", ct.c_double),
-                ("flags", ct.c_uint)]
+                ("drift_tol", ct.c_double), ("flags", ct.c_uint)]
 
 
 FMMH_LF_ACC_VALID = 1
@@ -105,6 +105,7 @@ SIGNATURES = {
     "fmmh_evaluate_frozen": (ct.c_int, [_vp, _sz, _pd, _pd, _pd, _pd]),
     "fmmh_evaluate_frozen_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
     "fmmh_leapfrog_default": (None, [ct.POINTER(_LeapOpts)]),
+    "fmmh_leapfrog_stats": (ct.c_int, [_vp, _pu64]),
     "fmmh_leapfrog_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp,
                                         ct.POINTER(_LeapOpts), _pd]),
     "fmmh_ic_plummer": (ct.c_int, [_sz, ct.c_double, ct.c_uint64, _pd, _pd, _pd]),
@@ -268,13 +269,14 @@ class Fmm:
 
     def leapfrog_device(self, n, pos_ptr, vel_ptr, mass_ptr, acc_ptr, pot_ptr, dt, steps,
                         refresh_every=1, dt_eta=0.0, dt_eps_len=0.02, accel_cap=0.0,
-                        acc_valid=False, graph=False):
+                        acc_valid=False, graph=False, drift_tol=0.25):
         """Kick-drift-kick steps on device-resident id-order state
         (sim.rs:539-594); returns the time advanced."""
         o = _LeapOpts()
         lib().fmmh_leapfrog_default(ct.byref(o))
         o.dt, o.steps, o.refresh_every = float(dt), int(steps), int(refresh_every)
         o.dt_eta, o.dt_eps_len, o.accel_cap = float(dt_eta), float(dt_eps_len), float(accel_cap)
+        o.drift_tol = float(drift_tol)
         o.flags = (FMMH_LF_ACC_VALID if acc_valid else 0) | (FMMH_LF_GRAPH if graph else 0)
         t = ct.c_double(0.0)
         self._check(lib().fmmh_leapfrog_device(self._h, n, _vp(pos_ptr), _vp(vel_ptr),
@@ -282,6 +284,12 @@ class Fmm:
                                               ct.byref(o), ct.byref(t)))
         self.n = n
         return t.value
+
+    def leapfrog_stats(self):
+        out = np.zeros(4, np.uint64)
+        self._check(lib().fmmh_leapfrog_stats(self._h, _p(out, _pu64)))
+        return dict(zip(["steps", "rebuilds", "frozen_steps", "drift_rebuilds"],
+                        (int(v) for v in out)))
 
     # ---- parity views
     def tree(self):

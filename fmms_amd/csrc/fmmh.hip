@@ -112,7 +112,7 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
-                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state};
+                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -507,15 +507,16 @@ struct fmmh_ctx {
   }
 
   // ----------------------------------------------------------- leapfrog
-  Buf lf_part, lf_state;
+  Buf lf_part, lf_state, lf_flag;
+  uint64_t lf_stats[4] = {0, 0, 0, 0};  // steps, rebuilds, frozen steps, drift-forced rebuilds
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   struct GraphKey {
     const void *pos, *vel, *mass, *acc, *pot;
-    double dt, eta, eps_len, cap;
+    double dt, eta, eps_len, cap, tol;
     bool operator==(const GraphKey& o) const {
       return pos == o.pos && vel == o.vel && mass == o.mass && acc == o.acc && pot == o.pot &&
-             dt == o.dt && eta == o.eta && eps_len == o.eps_len && cap == o.cap;
+             dt == o.dt && eta == o.eta && eps_len == o.eps_len && cap == o.cap && tol == o.tol;
     }
   } gkey{};
 
@@ -533,6 +534,9 @@ struct fmmh_ctx {
     launch_step_dt(acc, n, o.dt, o.dt_eta, o.dt_eps_len, part, st, s);
     launch_kick_drift(n, pos, vel, acc, st, s);
     evaluate_frozen(pos, mass, acc, pot);
+    if (o.drift_tol > 0.0)
+      launch_drift_check((uint32_t)n, (const uint32_t*)leaf_of_slot.p, cells(), bodies(),
+                         (const BBox*)bbox.p, o.drift_tol, (uint32_t*)lf_flag.p, s);
     if (o.accel_cap > 0.0) launch_cap_accel(n, acc, o.accel_cap, s);
     launch_kick(n, vel, acc, st, s);
   }
@@ -554,36 +558,52 @@ struct fmmh_ctx {
     }
     graph = g;
     HCHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    gkey = GraphKey{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap};
+    gkey = GraphKey{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap,
+                    o.drift_tol};
   }
 
   double leapfrog(size_t n_, double* pos, double* vel, const double* mass, double* acc,
                   double* pot, const fmmh_leapfrog_opts& o) {
     double* part = ensure<double>(lf_part, dt_scratch_words() * 8);
     double* st = ensure<double>(lf_state, 64);
+    uint32_t* flag = ensure<uint32_t>(lf_flag, 64);
     launch_fill_u64(st, 2, 0, s);
+    bool drifted = false;
     auto full = [&] {
       build(n_, pos, mass);
       evaluate(pos, mass, acc, pot, false);
       if (o.accel_cap > 0.0) launch_cap_accel(n, acc, o.accel_cap, s);
+      launch_fill_u64(flag, 1, 0, s);
+      drifted = false;
+      lf_stats[1]++;
     };
     if (!(o.flags & FMMH_LF_ACC_VALID) || !lists_ok || n != n_) full();
     const bool use_graph = (o.flags & FMMH_LF_GRAPH) != 0;
-    const GraphKey key{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap};
+    const GraphKey key{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap,
+                       o.drift_tol};
     for (int k = 0; k < o.steps; k++) {
-      if (tree_evals >= (uint32_t)o.refresh_every) {
+      lf_stats[0]++;
+      if (drifted || tree_evals >= (uint32_t)o.refresh_every) {
+        if (drifted && tree_evals < (uint32_t)o.refresh_every) lf_stats[3]++;
         launch_step_dt(acc, n, o.dt, o.dt_eta, o.dt_eps_len, part, st, s);
         launch_kick_drift(n, pos, vel, acc, st, s);
         full();
         launch_kick(n, vel, acc, st, s);
-      } else if (use_graph) {
+        continue;
+      }
+      if (use_graph) {
         if (!gexec || !(gkey == key)) capture_frozen_step(pos, vel, mass, acc, pot, o, part, st);
         HCHECK(hipGraphLaunch(gexec, s));
         tree_evals++;
       } else {
         frozen_step(pos, vel, mass, acc, pot, o, part, st);
       }
+      lf_stats[2]++;
+      // a body left its frozen leaf: the next step rebuilds the tree and lists
+      if (o.drift_tol > 0.0 && k + 1 < o.steps) drifted = read1(flag) != 0;
     }
+    if (o.drift_tol > 0.0) drifted = read1(flag) != 0;
+    if (drifted) tree_evals = (uint32_t)o.refresh_every;  // carried into the next call
     double* h = (double*)host_stage(16);
     HCHECK(hipMemcpyAsync(h, st, 16, hipMemcpyDeviceToHost, s));
     HCHECK(hipStreamSynchronize(s));
@@ -831,7 +851,14 @@ void fmmh_leapfrog_default(fmmh_leapfrog_opts* o) {
   o->dt_eta = 0.0;
   o->dt_eps_len = 0.02;
   o->accel_cap = 0.0;
+  o->drift_tol = 0.25;
   o->flags = 0;
+}
+
+int fmmh_leapfrog_stats(const fmmh_ctx* ctx, uint64_t* out4) {
+  if (!ctx || !out4) return FMMH_E_STATE;
+  for (int i = 0; i < 4; i++) out4[i] = ctx->lf_stats[i];
+  return FMMH_OK;
 }
 
 int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
@@ -844,10 +871,11 @@ int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
   if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
   if (!(o->dt > 0.0) || !std::isfinite(o->dt) || o->steps < 0 || o->refresh_every < 1 ||
       !(o->dt_eta >= 0.0) || !(o->dt_eps_len > 0.0) || !(o->accel_cap >= 0.0) ||
+      !(o->drift_tol >= 0.0) ||
       (o->flags & ~(FMMH_LF_ACC_VALID | FMMH_LF_GRAPH)))
     return fail(ctx, FMMH_E_PARAM,
                 "leapfrog options: dt > 0, steps >= 0, refresh_every >= 1, dt_eta >= 0, "
-                "dt_eps_len > 0, accel_cap >= 0, known flags only");
+                "dt_eps_len > 0, accel_cap >= 0, drift_tol >= 0, known flags only");
   return guarded(ctx, [&] {
     const double t = ctx->leapfrog(n, d_pos, d_vel, d_mass, d_accel, d_pot, *o);
     if (time_advanced) *time_advanced = t;

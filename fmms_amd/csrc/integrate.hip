@@ -96,6 +96,23 @@ __global__ void __launch_bounds__(IT) k_cap_accel(size_t n, double* __restrict__
   }
 }
 
+// Drift monitor of frozen steps: flag = 1 when a body (slot order, positions
+// just gathered) lies farther from its leaf's frozen centre than
+// rho + tol * max(rho, side / 2), side = the leaf's octant edge at build.
+__global__ void __launch_bounds__(IT) k_drift_check(uint32_t n, const uint32_t* __restrict__ leaf_of_slot,
+                                                   DevCells c, DevBodies b,
+                                                   const BBox* __restrict__ bb, double tol,
+                                                   uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * IT + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t L = leaf_of_slot[i];
+  const double dx = b.x[i] - c.cx[L], dy = b.y[i] - c.cy[L], dz = b.z[i] - c.cz[L];
+  const double rho = c.rho[L];
+  const double side = ldexp(bb->side, -(int)c.level[L]);
+  const double lim = rho + tol * fmax(rho, 0.5 * side);
+  if (dx * dx + dy * dy + dz * dz > lim * lim) flag[0] = 1u;
+}
+
 __global__ void k_fill_u64(uint64_t* __restrict__ p, size_t words, uint64_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
        i += (size_t)gridDim.x * blockDim.x)
@@ -127,6 +144,12 @@ void launch_kick(size_t n, double* vel, const double* acc, const double* st, hip
 
 void launch_cap_accel(size_t n, double* acc, double cap, hipStream_t s) {
   hipLaunchKernelGGL(k_cap_accel, dim3((n + IT - 1) / IT), dim3(IT), 0, s, n, acc, cap);
+}
+
+void launch_drift_check(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
+                        const BBox* bb, double tol, uint32_t* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_drift_check, dim3((n + IT - 1) / IT), dim3(IT), 0, s, n, leaf_of_slot, c, b,
+                     bb, tol, flag);
 }
 
 void launch_fill_u64(void* p, size_t words, uint64_t v, hipStream_t s) {

@@ -176,6 +176,8 @@ void launch_kick_drift(size_t n, double* pos, double* vel, const double* acc, co
 void launch_kick(size_t n, double* vel, const double* acc, const double* st, hipStream_t s);
 void launch_cap_accel(size_t n, double* acc, double cap, hipStream_t s);
 void launch_fill_u64(void* p, size_t words, uint64_t v, hipStream_t s);
+void launch_drift_check(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
+                        const BBox* bb, double tol, uint32_t* flag, hipStream_t s);
 
 // ------------------------------------------------------------ direct.hip
 void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,

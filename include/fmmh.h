@@ -115,7 +115,12 @@ int fmmh_evaluate_frozen_device(fmmh_ctx* ctx, size_t n, const double* d_pos,
  *   v += a dt/2.
  * The tree and lists are rebuilt when they have served refresh_every
  * evaluations (1 = every step, as the energy gate does); steps in between are
- * frozen evaluations. With FMMH_LF_GRAPH the frozen step (dt, kicks, drift and
+ * frozen evaluations. Drift monitor (drift_tol > 0): a frozen step that
+ * leaves any body farther than rho + drift_tol * max(rho, side/2) from its
+ * leaf's frozen centre (rho, side: the leaf's radius and octant edge at the
+ * build) makes the next step rebuild — expansions about frozen centres
+ * diverge once bodies leave their cells (the reference's age rule,
+ * gpufmm.rs:14-20, bounds the same hazard by step count only). With FMMH_LF_GRAPH the frozen step (dt, kicks, drift and
  * the whole force evaluation) is captured once per tree into a hipGraph and
  * replayed. FMMH_LF_ACC_VALID: accel/pot already hold a(x) of the entry
  * positions from this context's last evaluation (else they are evaluated
@@ -131,9 +136,13 @@ typedef struct fmmh_leapfrog_opts {
   double dt_eta;     /* 0: fixed dt; sim.rs uses 0.25 */
   double dt_eps_len; /* sim.rs uses 0.02 */
   double accel_cap;  /* 0: off; sim.rs uses 50 */
+  double drift_tol;  /* > 0: drift monitor (default 0.25), 0: off */
   unsigned flags;
 } fmmh_leapfrog_opts;
 void fmmh_leapfrog_default(fmmh_leapfrog_opts* out);
+/* Counters of this context's leapfrog calls: [steps, tree rebuilds, frozen
+ * steps, rebuilds forced by the drift monitor]. */
+int fmmh_leapfrog_stats(const fmmh_ctx* ctx, uint64_t* out4);
 int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
                          const double* d_mass, double* d_accel, double* d_pot,
                          const fmmh_leapfrog_opts* opts, double* time_advanced);

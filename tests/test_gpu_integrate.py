@@ -201,3 +201,28 @@ def test_energy_gate_smoke(fa, refresh_every, graph):
     assert mom < 1e-12                 # :246
     assert abs(drift) < 1e-2           # :247
     assert max_dev < 1e-2              # :248
+
+
+def test_drift_monitor_forces_rebuilds(fa):
+    s = fa.ic_plummer(20000, rmax=20.0, seed=65)
+    dt = 2e-2  # bodies cross core leaves within a few steps
+    say("[leapfrog] drift monitor")
+    g = fa.Fmm(None, None, p=6, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+    it = fa.Integrator(g, s.pos, s.vel, s.mass)
+    it.step(dt, steps=8, refresh_every=1000, drift_tol=0.25)
+    st = g.leapfrog_stats()
+    assert st["steps"] == 8 and st["drift_rebuilds"] >= 1
+    assert st["rebuilds"] == 1 + st["drift_rebuilds"]
+    assert st["frozen_steps"] == 8 - st["drift_rebuilds"]
+    x, _, a, _ = it.state()
+    # the forces of the last step stay close to a fresh evaluation's
+    ref = fa.Fmm(None, None, p=6, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+    r = ref.accel(x, s.mass)
+    rel = np.linalg.norm(a - r.accel, axis=1) / np.linalg.norm(r.accel, axis=1)
+    assert np.percentile(rel, 99) < 1e-3
+    # monitor off: the tree is never rebuilt within refresh_every
+    g2 = fa.Fmm(None, None, p=6, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+    it2 = fa.Integrator(g2, s.pos, s.vel, s.mass)
+    it2.step(dt, steps=3, refresh_every=1000, drift_tol=0.0)
+    st2 = g2.leapfrog_stats()
+    assert st2["rebuilds"] == 1 and st2["frozen_steps"] == 3
