@@ -46,6 +46,7 @@ SOURCES = {
     "fmmh.hip": "off",
     "ic.hip": "off",
     "integrate.hip": "off",
+    "soft.hip": "fast",
 }
 
 

@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
                                            const double* __restrict__ by,
                                            const double* __restrict__ bz,
                                            const cplx* __restrict__ locals,
+                                           const double* __restrict__ lap,
                                            double4* __restrict__ far) {
   constexpr int D = hdim(P), NF = hnpos(P);
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,7 +191,15 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
       U1[q] = U[q];
     }
   }
-  far[k] = make_double4(-p11r, -p11i, -p10r, p00);
+  double4 o = make_double4(-p11r, -p11i, -p10r, p00);
+  if (lap) {  // target-side trace term of the softened far field (x = centre - body)
+    const double L = lap[c];
+    o.x -= x * L * (1.0 / 3.0);
+    o.y -= y * L * (1.0 / 3.0);
+    o.z -= z * L * (1.0 / 3.0);
+    o.w += r2 * L * (1.0 / 6.0);
+  }
+  far[k] = o;
 }
 
 template <int P>
@@ -201,9 +210,9 @@ static void l2l_t(const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
 
 template <int P>
 static void l2p_t(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                  const cplx* locals, double4* far, hipStream_t s) {
+                  const cplx* locals, const double* lap, double4* far, hipStream_t s) {
   hipLaunchKernelGGL(k_l2p<P>, dim3((n + 63) / 64), dim3(64), 0, s, n, leaf_of_slot, c.cx, c.cy,
-                     c.cz, b.x, b.y, b.z, locals, far);
+                     c.cz, b.x, b.y, b.z, locals, lap, far);
 }
 
 #define FMMH_P_SWITCH2(p, CALL) \
@@ -233,9 +242,9 @@ bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* lo
 }
 
 bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                const cplx* locals, double4* far, hipStream_t s) {
+                const cplx* locals, const double* lap, double4* far, hipStream_t s) {
   if (n == 0) return true;
-#define CALL(P) l2p_t<P>(n, leaf_of_slot, c, b, locals, far, s)
+#define CALL(P) l2p_t<P>(n, leaf_of_slot, c, b, locals, lap, far, s)
   FMMH_P_SWITCH2(p, CALL)
 #undef CALL
   return true;

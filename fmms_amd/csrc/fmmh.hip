@@ -97,6 +97,7 @@ struct fmmh_ctx {
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
   Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
+  Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2 and Lap Phi per cell
   int rot_p = -1;
   std::vector<char> rot_struct;
   Buf direct_ids, direct_out;
@@ -112,7 +113,7 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
-                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag};
+                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -352,6 +353,13 @@ struct fmmh_ctx {
         launch_m2m_gen(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom,
                        s);
     }
+    if (prm.softening > 0.0) {  // trace moments of the softened far field
+      double* q = (double*)ctrace.p;
+      launch_trace_leaf((uint32_t*)leaf_ids.p, nleaves, c, b, q, s);
+      for (int L = (int)max_level - 1; L >= 0; L--)
+        launch_trace_up((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, p, mom,
+                        q, s);
+    }
   }
 
   // Multipole powers, f pre-pass and fmin (error-controlled MAC only).
@@ -450,6 +458,12 @@ struct fmmh_ctx {
                      loc, s);
       mark(ST_M2L_REDUCE);
     }
+    double* lap = nullptr;
+    if (soft2 > 0.0) {
+      lap = (double*)clap.p;
+      launch_m2l_trace(nc, (const uint32_t*)tptr.p, msorted, sbits, c, (const double*)ctrace.p,
+                       soft2, p, loc, lap, s);
+    }
     mark(ST_L2L);
     for (int L = 1; L <= (int)max_level; L++) {
       const uint32_t* lc = (uint32_t*)lvl_cells.p + lvl_off[L];
@@ -458,12 +472,13 @@ struct fmmh_ctx {
         launch_l2l(p, lc, cnt, c, loc, s);
       else
         launch_l2l_gen(p, lc, cnt, c, loc, s);
+      if (lap) launch_lap_down(lc, cnt, c.parent, lap, s);
     }
     mark(ST_L2P);
     if (!gen)
-      launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, (double4*)far.p, s);
+      launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, lap, (double4*)far.p, s);
     else
-      launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, (double4*)far.p, s);
+      launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
     mark(ST_P2P);
     launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
                b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
@@ -477,6 +492,10 @@ struct fmmh_ctx {
     drop_graph();
     ensure(moments, nc * D * sizeof(cplx));
     ensure(locals, nc * D * sizeof(cplx));
+    if (prm.softening > 0.0) {
+      ensure(ctrace, nc * 8);
+      ensure(clap, nc * 8);
+    }
     upward(pos, mass, regather);
     mark(ST_PREPASS);
     const bool use_err = std::isfinite(prm.eps_mac);

@@ -27,11 +27,15 @@ __device__ __forceinline__ cplx cmul_(cplx a, cplx b) {
 }
 
 // Column m (both signs) of Theta(r) of the (optionally Plummer-softened)
-// kernel: the harmonic.rs:91-129 recursion with r^2 -> r^2 + soft2.
+// kernel: the harmonic.rs:91-129 recursion; softened, the harmonic
+// derivatives of (r^2 + eps^2)^{-1/2} = those of 1/r times (r/s)^{2n+1}, i.e.
+// 1/r^2 -> 1/s^2 and the (n-2) term scaled by r^2/s^2.
 __device__ void theta_column_gen(int p, double x, double y, double z, double soft2, int m,
                                  cplx* tab) {
-  const double r2 = x * x + y * y + z * z + soft2;
+  const double rr2 = x * x + y * y + z * z;
+  const double r2 = rr2 + soft2;
   const double ir2 = 1.0 / r2;
+  const double ks = soft2 > 0.0 ? rr2 * ir2 : 1.0;
   cplx t = cplx{1.0 / sqrt(r2), 0.0};
   for (int j = 1; j <= m; j++) {
     const double f = (double)(2 * j - 1) * ir2;
@@ -45,7 +49,7 @@ __device__ void theta_column_gen(int p, double x, double y, double z, double sof
       tn = t;
     } else {
       const double a = (double)(2 * n - 1) * z;
-      const double b = (double)((n - 1) * (n - 1) - m * m);
+      const double b = (double)((n - 1) * (n - 1) - m * m) * ks;
       tn = cplx{(a * t1.re - b * t2.re) * ir2, (a * t1.im - b * t2.im) * ir2};
       t2 = t1;
       t1 = tn;
@@ -150,13 +154,10 @@ __global__ void __launch_bounds__(GEN_T) k_m2l_rot(int p, uint32_t nc,
       phb[i] = cplx{cb, sb};
     }
     if (threadIdx.x == 0) {
-      if (soft2 > 0.0) {  // axial Theta_j^0 of the softened kernel (same recursion)
+      if (soft2 > 0.0) {  // axial harmonic derivatives of the softened kernel, q! R^q / s^{2q+1}
         const double R = sqrt(r2), s2 = r2 + soft2;
         facr[0] = 1.0 / sqrt(s2);
-        for (int q = 1; q <= p; q++) {
-          const double prev2 = q >= 2 ? facr[q - 2] : 0.0;
-          facr[q] = ((double)(2 * q - 1) * R * facr[q - 1] - (double)((q - 1) * (q - 1)) * prev2) / s2;
-        }
+        for (int q = 1; q <= p; q++) facr[q] = facr[q - 1] * ((double)q * R / s2);
       } else {  // (n+k)!/R^(n+k+1)
         const double rinv = 1.0 / sqrt(r2);
         double irp = rinv;
@@ -308,6 +309,7 @@ constexpr int GEN_PMAX = FMMH_MAX_P;
 __global__ void __launch_bounds__(64) k_l2p_gen(int p, const uint32_t* __restrict__ leaf_ids,
                                                DevCells c, DevBodies b,
                                                const cplx* __restrict__ locals,
+                                               const double* __restrict__ lap,
                                                double4* __restrict__ far) {
   extern __shared__ cplx F[];
   const int D = hdim(p);
@@ -359,7 +361,15 @@ __global__ void __launch_bounds__(64) k_l2p_gen(int p, const uint32_t* __restric
         U1[q] = U[q];
       }
     }
-    far[k] = make_double4(-p11r, -p11i, -p10r, p00);
+    double4 o = make_double4(-p11r, -p11i, -p10r, p00);
+    if (lap) {  // target-side trace term of the softened far field (x = centre - body)
+      const double L = lap[leaf];
+      o.x -= x * L * (1.0 / 3.0);
+      o.y -= y * L * (1.0 / 3.0);
+      o.z -= z * L * (1.0 / 3.0);
+      o.w += r2 * L * (1.0 / 6.0);
+    }
+    far[k] = o;
   }
 }
 
@@ -454,10 +464,10 @@ void launch_l2l_gen(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx
 }
 
 void launch_l2p_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
-                    const cplx* locals, double4* far, hipStream_t s) {
+                    const cplx* locals, const double* lap, double4* far, hipStream_t s) {
   if (nleaves == 0) return;
   hipLaunchKernelGGL(k_l2p_gen, dim3(nleaves), dim3(64), (size_t)hdim(p) * sizeof(cplx), s, p,
-                     leaf_ids, c, b, locals, far);
+                     leaf_ids, c, b, locals, lap, far);
 }
 
 }  // namespace fmmh

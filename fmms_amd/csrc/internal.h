@@ -147,13 +147,25 @@ void launch_m2l_gen(int p, uint32_t nc, const uint32_t* tptr, const uint64_t* sk
 void launch_l2l_gen(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
                     hipStream_t s);
 void launch_l2p_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
-                    const cplx* locals, double4* far, hipStream_t s);
+                    const cplx* locals, const double* lap, double4* far, hipStream_t s);
 
 // ---------------------------------------------------------- downward.hip
 bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
                 hipStream_t s);
 bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                const cplx* locals, double4* far, hipStream_t s);
+                const cplx* locals, const double* lap, double4* far, hipStream_t s);
+
+// --------------------------------------------------------------- soft.hip
+// Trace terms of the softened far field (lap may be null: eps = 0).
+void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                       double* q, hipStream_t s);
+void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, const cplx* mom,
+                     double* q, hipStream_t s);
+void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
+                      DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
+                      hipStream_t s);
+void launch_lap_down(const uint32_t* cells, uint32_t cnt, const int32_t* parent, double* lap,
+                     hipStream_t s);
 
 // -------------------------------------------------------------- near.hip
 void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,

@@ -31,11 +31,15 @@ __device__ __forceinline__ void cmac(cplx& f, double ar, double ai, double br, d
 template <int P>
 __device__ __forceinline__ void m2l_accumulate(double x, double y, double z, double soft2,
                                                const cplx* M, cplx* F) {
-  // Softening: the recursion with r^2 -> r^2 + eps^2 gives the traceless
-  // derivatives of the Plummer kernel (soft2 = 0: pure 1/r).
-  const double r2 = x * x + y * y + z * z + soft2;
+  // Softening: the harmonic derivatives of the Plummer kernel are those of
+  // 1/r times (r/s)^{2n+1}, s^2 = r^2 + eps^2; in the recursion that is
+  // 1/r^2 -> 1/s^2 and the (n-2) term scaled by ks = r^2/s^2 (soft2 = 0:
+  // ks = 1, pure 1/r).
+  const double rr2 = x * x + y * y + z * z;
+  const double r2 = rr2 + soft2;
   const double ir = rsqrt(r2);
   const double ir2 = ir * ir;
+  const double ks = soft2 > 0.0 ? rr2 * ir2 : 1.0;
   cplx T[P + 1], T1[P + 1], T2[P + 1];
 #pragma unroll
   for (int s = 0; s <= P; s++) {
@@ -48,7 +52,7 @@ __device__ __forceinline__ void m2l_accumulate(double x, double y, double z, dou
       for (int q = 0; q <= P; q++) {
         if (q > s) continue;
         if (q + 2 <= s) {
-          const double bq = (double)((s - 1) * (s - 1) - q * q);
+          const double bq = (double)((s - 1) * (s - 1) - q * q) * ks;
           T[q].re = (a * T1[q].re - bq * T2[q].re) * ir2;
           T[q].im = (a * T1[q].im - bq * T2[q].im) * ir2;
         } else if (q == s - 1) {

@@ -207,10 +207,6 @@ size_t build_rec(const std::vector<uint64_t>& keys, std::vector<Cell>& cells, ui
 void theta_into(const double r[3], int p, C* t, double eps2 = 0.0) {
   double x = r[0], y = r[1], z = r[2];
   double r2 = x * x + y * y + z * z;
-  // Softening extension: the Plummer kernel (r^2+eps^2)^(-1/2) is a function
-  // of r^2+eps^2, so the same recursion with r^2 -> r^2+eps^2 yields its
-  // traceless derivatives. eps2 == 0 leaves the reference arithmetic intact.
-  if (eps2 > 0.0) r2 = r2 + eps2;
   double ir = 1.0 / std::sqrt(r2);
   t[0] = C{ir, 0.0};
   for (int n = 1; n <= p; n++) {
@@ -233,6 +229,17 @@ void theta_into(const double r[3], int p, C* t, double eps2 = 0.0) {
   for (int m = 1; m <= p; m++) {
     double sgn = (m % 2 == 0) ? 1.0 : -1.0;
     for (int n = m; n <= p; n++) t[idx(n, -m)] = cscale(cconj(t[idx(n, m)]), sgn);
+  }
+  // Softening extension: the harmonic (traceless) derivatives of the Plummer
+  // kernel (r^2 + eps^2)^{-1/2} are those of 1/r times (r/s)^{2n+1},
+  // s^2 = r^2 + eps^2 (Hobson's theorem for a function of r^2: Y_n(grad) F(r^2)
+  // = 2^n Y_n(r) F^(n)(r^2)). eps2 == 0 leaves the reference arithmetic intact.
+  if (eps2 > 0.0) {
+    double q = r2 / (r2 + eps2), w = std::sqrt(q);
+    for (int n = 0; n <= p; n++) {
+      for (int m = -n; m <= n; m++) t[idx(n, m)] = cscale(t[idx(n, m)], w);
+      w *= q;
+    }
   }
 }
 
@@ -550,14 +557,11 @@ struct RotCtx {  // rotation.rs:106-253
     double s_beta = std::sqrt(r[0] * r[0] + r[1] * r[1]);
     double beta = std::atan2(s_beta, r[2]);
     if (eps2 > 0.0) {
-      // softening extension: axial Theta_j^0(0,0,R) of the Plummer kernel
-      // by the theta recursion with R^2 -> R^2 + eps^2 (see theta_into).
+      // softening extension: axial harmonic derivatives of the Plummer kernel,
+      // j! R^j / s^{2j+1} (j!/R^{j+1} times (R/s)^{2j+1}, see theta_into)
       double R = std::sqrt(r2), s2 = r2 + eps2;
       facr[0] = 1.0 / std::sqrt(s2);
-      for (int j = 1; j <= p; j++) {
-        double prev2 = j >= 2 ? facr[j - 2] : 0.0;
-        facr[j] = ((double)(2 * j - 1) * R * facr[j - 1] - (double)((j - 1) * (j - 1)) * prev2) / s2;
-      }
+      for (int j = 1; j <= p; j++) facr[j] = facr[j - 1] * ((double)j * R / s2);
     } else {
       double irp = rinv;
       for (int j = 0; j <= p; j++) {
@@ -701,7 +705,7 @@ struct orc_ctx {
   // last run
   int p = 0;
   std::vector<C> moments, locals_walk, locals;
-  std::vector<double> pows, f, fmin;
+  std::vector<double> pows, f, fmin, lap;
   std::vector<uint32_t> m2l, p2p;
   Bodies bodies;
   double times[5] = {0, 0, 0, 0, 0};
@@ -1081,6 +1085,26 @@ void orc_binomials(int p, double* out) {
   std::memcpy(out, b.data(), b.size() * sizeof(double));
 }
 
+static void trace_pair(int p, const double* zs, const double* zt, double q, double mass,
+                       double soft2, C* f, double* lap);
+static void l2p_trace(const double* cen, const double* x, double L, double* pot, double* a);
+
+void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], const double* ma,
+                         const double* mb, double eps2, double* la, double* lb) {
+  M2lScratch s(p);
+  m2l_mutual(p, cen_a, cen_b, (const C*)ma, (const C*)mb, (C*)la, (C*)lb, s, eps2);
+}
+
+void orc_soft_trace_pair(int p, const double cen_src[3], const double cen_dst[3], double q,
+                         double mass, double eps2, double* f_dst, double* lap_dst) {
+  trace_pair(p, cen_src, cen_dst, q, mass, eps2, (C*)f_dst, lap_dst);
+}
+
+void orc_l2p_trace(const double cen[3], const double pos[3], double lap, double* pot,
+                   double acc[3]) {
+  l2p_trace(cen, pos, lap, pot, acc);
+}
+
 void orc_m2l_mutual(int p, const double cen_a[3], const double cen_b[3], const double* ma,
                     const double* mb, double* la, double* lb) {
   M2lScratch s(p);
@@ -1240,6 +1264,72 @@ void orc_get_bbox(const orc_ctx* t, double lo[3], double hi[3], double* inv_side
   std::memcpy(lo, t->lo, 3 * sizeof(double));
   std::memcpy(hi, t->hi, 3 * sizeof(double));
   *inv_side = t->inv_side;
+}
+
+// Softening extension, far field: the Taylor expansions of the Plummer
+// kernel K = (r^2 + eps^2)^{-1/2} about the source and the target centre
+// have, beside the harmonic (solid-harmonic) parts that M2L and L2P carry,
+// trace terms |d|^{2k} Y_{n-2k}(d) Lap^k K. The leading ones are kept:
+//  * source side, (q/6) Lap K with q = sum m |x - z|^2 and
+//    Lap K = -3 eps^2 s^{-5}: added to every M2L target's local expansion to
+//    first order (value into F_0^0, gradient into F_1^m; l2p_into gives
+//    pot = F00 - F10 dz - Re F11 dx - Im F11 dy);
+//  * target side, (|dx|^2 / 6) Lap Phi: lap[cell] = sum over its M2L
+//    sources of M Lap K at the centre, inherited by the children (L2L) and
+//    applied at L2P as pot += |dx|^2 lap / 6, a += dx lap / 3.
+// q is summed directly from the bodies of each cell.
+// One directed M2L pair's trace terms: source-side correction into the
+// target's local expansion f, Lap of the source's monopole field into *lap.
+static void trace_pair(int p, const double* zs, const double* zt, double q, double mass,
+                       double soft2, C* f, double* lap) {
+  double r[3] = {zt[0] - zs[0], zt[1] - zs[1], zt[2] - zs[2]};
+  double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + soft2;
+  double is = 1.0 / std::sqrt(s2);
+  double is5 = is * is * is * is * is;
+  *lap += -3.0 * soft2 * mass * is5;
+  double c = 0.5 * q * soft2 * is5;  // -(q/6) Lap K at zt
+  double g[3] = {5.0 * c * r[0] / s2, 5.0 * c * r[1] / s2, 5.0 * c * r[2] / s2};  // its gradient
+  f[idx(0, 0)].re -= c;
+  if (p >= 1) {
+    f[idx(1, 0)].re -= g[2];
+    f[idx(1, 1)].re -= g[0];
+    f[idx(1, 1)].im -= g[1];
+    f[idx(1, -1)].re += g[0];
+    f[idx(1, -1)].im -= g[1];
+  }
+}
+
+// Target-side trace term at L2P: pot += |dx|^2 L / 6, a += dx L / 3.
+static void l2p_trace(const double* cen, const double* x, double L, double* pot, double* a) {
+  double dx = x[0] - cen[0], dy = x[1] - cen[1], dz = x[2] - cen[2];
+  *pot += (dx * dx + dy * dy + dz * dz) * L / 6.0;
+  a[0] += dx * L / 3.0;
+  a[1] += dy * L / 3.0;
+  a[2] += dz * L / 3.0;
+}
+
+static void softened_trace(orc_ctx& t, int p, double soft2) {
+  const size_t nc = t.cells.size(), d = dimp(p);
+  std::vector<double> q(nc, 0.0);
+  for (size_t c = 0; c < nc; c++) {
+    const Cell& cell = t.cells[c];
+    double acc = 0.0;
+    for (size_t k = cell.first; k < cell.first + cell.count; k++) {
+      double dx = t.bodies.x[k] - cell.cen[0], dy = t.bodies.y[k] - cell.cen[1],
+             dz = t.bodies.z[k] - cell.cen[2];
+      acc += t.bodies.m[k] * (dx * dx + dy * dy + dz * dz);
+    }
+    q[c] = acc;
+  }
+  t.lap.assign(nc, 0.0);
+  auto add = [&](size_t src, size_t dst) {
+    trace_pair(p, t.cells[src].cen, t.cells[dst].cen, q[src], t.cells[src].mass, soft2,
+               t.locals.data() + dst * d, &t.lap[dst]);
+  };
+  for (size_t k = 0; k + 1 < t.m2l.size(); k += 2) {
+    add(t.m2l[k], t.m2l[k + 1]);
+    add(t.m2l[k + 1], t.m2l[k]);
+  }
 }
 
 // fmm.rs:69-142
@@ -1435,6 +1525,8 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
       }
     });
   }
+  t->lap.clear();
+  if (soft2 > 0.0 && !t->m2l.empty()) softened_trace(*t, p, soft2);
   double t3 = now_ms();
   t->locals_walk = t->locals;
   // l2l_down (translate.rs:209-252): ascending level
@@ -1451,6 +1543,7 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
           if (ch < 0) continue;
           l2l_into(p, cell.cen, t->cells[ch].cen, t->locals.data() + c * d,
                    t->locals.data() + (size_t)ch * d, u.data());
+          if (!t->lap.empty()) t->lap[ch] += t->lap[c];
         }
       }
     });
@@ -1470,6 +1563,7 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
         double a[3] = {0.0, 0.0, 0.0};
         double x[3] = {t->bodies.x[k], t->bodies.y[k], t->bodies.z[k]};
         l2p_into(p, cell.cen, x, t->locals.data() + c * d, &p0, a, u.data());
+        if (!t->lap.empty()) l2p_trace(cell.cen, x, t->lap[c], &p0, a);
         pot[id] += t->bodies.pot[k] + p0;
         accel[3 * id + 0] += t->bodies.ax[k] + a[0];
         accel[3 * id + 1] += t->bodies.ay[k] + a[1];

@@ -47,6 +47,9 @@ def _load():
         "orc_binomials": (None, [ct.c_int, _pd]),
         "orc_m2l_mutual": (None, [ct.c_int, _pd, _pd, _pd, _pd, _pd, _pd]),
         "orc_m2l_rotation": (None, [ct.c_int, _pd, _pd, _pd, _pd, _pd, _pd]),
+        "orc_m2l_mutual_soft": (None, [ct.c_int, _pd, _pd, _pd, _pd, _d, _pd, _pd]),
+        "orc_soft_trace_pair": (None, [ct.c_int, _pd, _pd, _d, _d, _d, _pd, _pd]),
+        "orc_l2p_trace": (None, [_pd, _pd, _d, _pd, _pd]),
         "orc_rot_blocksum": (_sz, [ct.c_int]),
         "orc_rot_swap": (None, [ct.c_int, _pd]),
         "orc_l2l": (None, [ct.c_int, _pd, _pd, _pd, _pd]),
@@ -254,6 +257,35 @@ def m2l_mutual(p, cen_a, cen_b, ma, mb):
 def m2l_rotation(p, cen_a, cen_b, ma, mb):
     """rotation.rs:262-435 — returns (locals_a, locals_b)."""
     return _m2l(lib().orc_m2l_rotation, p, cen_a, cen_b, ma, mb)
+
+
+def m2l_mutual_soft(p, cen_a, cen_b, ma, mb, eps2):
+    """Mutual M2L with the harmonic derivatives of the Plummer kernel."""
+    ma = np.ascontiguousarray(ma, dtype=np.complex128)
+    mb = np.ascontiguousarray(mb, dtype=np.complex128)
+    la = np.zeros(dim(p), np.complex128)
+    lb = np.zeros(dim(p), np.complex128)
+    a, b = _f64(cen_a), _f64(cen_b)
+    lib().orc_m2l_mutual_soft(p, _p(a), _p(b), ma.ctypes.data_as(_pd), mb.ctypes.data_as(_pd),
+                              float(eps2), la.ctypes.data_as(_pd), lb.ctypes.data_as(_pd))
+    return la, lb
+
+
+def soft_trace_pair(p, cen_src, cen_dst, q, mass, eps2, f_dst):
+    """Adds the source-side trace correction into f_dst (in place); returns
+    the Lap contribution of the source monopole at the target centre."""
+    lap = np.zeros(1)
+    a, b = _f64(cen_src), _f64(cen_dst)
+    lib().orc_soft_trace_pair(p, _p(a), _p(b), float(q), float(mass), float(eps2),
+                              f_dst.ctypes.data_as(_pd), _p(lap))
+    return lap[0]
+
+
+def l2p_trace(cen, pos, lap):
+    pot = np.zeros(1)
+    acc = np.zeros(3)
+    lib().orc_l2p_trace(_p(_f64(cen)), _p(_f64(pos)), float(lap), _p(pot), _p(acc))
+    return pot[0], acc
 
 
 def rot_swap(p):

@@ -480,3 +480,80 @@ def test_direct_targets_matches_direct():
     at, pt, _ = O.direct_targets(pos, mass, ids, softening=1e-3, nthreads=2)
     assert np.allclose(at, a[ids], rtol=1e-12, atol=1e-12)
     assert np.allclose(pt, pot[ids], rtol=1e-12)
+
+
+def _cluster(rng, nb, rad):
+    d = rng.normal(size=(nb, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d *= rad * rng.random(nb)[:, None] ** (1 / 3)
+    m = rng.random(nb) + 0.5
+    d -= (m[:, None] * d).sum(0) / m.sum()
+    return d, m
+
+
+@pytest.mark.parametrize("eps", [0.02, 0.05])
+def test_softened_far_field_against_exact_sum(eps):
+    """One well-separated pair of clusters (radii 0.15, separation 1, p = 8):
+    M2L with the harmonic derivatives of the Plummer kernel, the source-side
+    trace correction, L2P and the target-side trace term, against the exact
+    softened sum. Dropping the trace terms (a plausible omission) leaves the
+    leading O(eps^2 rho / r^3) error and fails the same bound; eps -> 0
+    reproduces the unsoftened M2L bit for bit."""
+    p = 8
+    rng = np.random.default_rng(11)
+    dA, mA = _cluster(rng, 300, 0.15)
+    dB, _ = _cluster(rng, 40, 0.15)
+    zA = np.zeros(3)
+    u = rng.normal(size=3)
+    zB = u / np.linalg.norm(u)
+    M = sum(mm * O.upsilon_table(dd, p) for mm, dd in zip(mA, dA))
+    q = float(np.sum(mA * (dA ** 2).sum(1)))
+    zero = np.zeros(O.dim(p), np.complex128)
+    _, F = O.m2l_mutual_soft(p, zA, zB, M, zero, eps * eps)
+    Fc = F.copy()
+    lap = O.soft_trace_pair(p, zA, zB, q, mA.sum(), eps * eps, Fc)
+    assert lap == pytest.approx(-3 * eps * eps * mA.sum() * (1 + eps * eps) ** -2.5, rel=1e-14)
+    err_with, err_without = [], []
+    for db in dB:
+        x = zB + db
+        diff = x - (zA + dA)
+        s = np.sqrt((diff ** 2).sum(1) + eps * eps)
+        a_ex = -(mA[:, None] * diff / s[:, None] ** 3).sum(0)
+        p_ex = np.sum(mA / s)
+        p0, a0 = O.l2p(p, zB, x, F)
+        p1, a1 = O.l2p(p, zB, x, Fc)
+        pt, at = O.l2p_trace(zB, x, lap)
+        p1, a1 = p1 + pt, a1 + at
+        err_with.append(max(np.linalg.norm(a1 - a_ex) / np.linalg.norm(a_ex),
+                            abs(p1 - p_ex) / p_ex))
+        err_without.append(np.linalg.norm(a0 - a_ex) / np.linalg.norm(a_ex))
+    bound = 0.6 * (eps / 0.05) ** 2 * 2e-4
+    assert np.median(err_with) < bound, (np.median(err_with), bound)
+    assert np.median(err_without) > bound
+    assert np.median(err_with) < 0.5 * np.median(err_without)
+    # eps -> 0: the unsoftened M2L exactly
+    _, F0 = O.m2l_mutual_soft(p, zA, zB, M, zero, 0.0)
+    _, Fu = O.m2l_mutual(p, zA, zB, M, zero)
+    assert np.array_equal(F0, Fu)
+
+
+def test_softened_fmm_error_matches_unsoftened_truncation():
+    """Full oracle FMM, Plummer N = 30,000, p = 8, theta = 0.5: with Plummer
+    softening 1e-3 the error against the softened direct sum stays at the
+    unsoftened truncation level (the softened far field adds no error of its
+    own at this eps)."""
+    n = 30000
+    pos, mass = O.plummer(n, seed=3)
+    ids = O.sample_ids(n, 400, 7)
+    meds = {}
+    for soft in (0.0, 1e-3):
+        f = O.OracleFmm(pos, mass, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=soft)
+        a, _ = f.run(pos, mass, nthreads=4)
+        acc = np.empty((len(ids), 3))
+        pot = np.empty(len(ids))
+        ff = np.empty(len(ids))
+        O.lib().orc_direct_targets(n, O._p(pos), O._p(mass), soft, len(ids),
+                                   O._p(ids, O._pu32), O._p(acc), O._p(pot), O._p(ff), 4)
+        rel = np.linalg.norm(a[ids] - acc, axis=1) / np.linalg.norm(acc, axis=1)
+        meds[soft] = np.median(rel)
+    assert meds[1e-3] < 1.3 * meds[0.0], meds
