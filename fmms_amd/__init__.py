@@ -35,6 +35,7 @@ FMMH_E_HIP = 6
 FMMH_E_STATE = 7
 FMMH_FLAG_TIMING = 1
 MAX_P = 40
+MAX_SYSTEMS = 4096
 
 _CODES = {1: "EMPTY", 2: "PARAM", 3: "OVER_BUDGET", 4: "OVERFULL_LEAF", 5: "NONFINITE", 6: "HIP",
           7: "STATE"}
@@ -102,6 +103,7 @@ SIGNATURES = {
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_plummer_trunc": (None, [_sz, _pd, ct.c_double, ct.c_uint64, _pd, _pd]),
+    "fmmh_set_systems": (ct.c_int, [_vp, _sz, _pu64]),
     "fmmh_evaluate_frozen": (ct.c_int, [_vp, _sz, _pd, _pd, _pd, _pd]),
     "fmmh_evaluate_frozen_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
     "fmmh_leapfrog_default": (None, [ct.POINTER(_LeapOpts)]),
@@ -250,6 +252,18 @@ class Fmm:
     def evaluate_device(self, n, pos_ptr, mass_ptr, acc_ptr, pot_ptr=0):
         self._check(lib().fmmh_evaluate_device(self._h, n, _vp(pos_ptr), _vp(mass_ptr), _vp(acc_ptr),
                                               _vp(pot_ptr) if pot_ptr else None))
+
+    # ---- batched ensembles
+    def set_systems(self, sizes):
+        """Treat the bodies as independent systems of the given sizes (in
+        input order): one forest, one launch per stage, no cross-system
+        interactions. `sizes` of length 1 restores single-system mode."""
+        sizes = np.asarray(sizes, dtype=np.uint64)
+        off = np.zeros(len(sizes) + 1, np.uint64)
+        off[1:] = np.cumsum(sizes)
+        self._check(lib().fmmh_set_systems(self._h, len(sizes), _p(off, _pu64)))
+        self.n = None
+        return off
 
     # ---- temporal reuse and time integration
     def run_frozen(self, pos, mass):

@@ -66,6 +66,12 @@ struct fmmh_ctx {
   std::string err;
   size_t used = 0;
   std::vector<Buf*> bufs;
+  // ensemble layout: nsys systems with body ranges sys_off[s]..sys_off[s+1];
+  // vlev virtual tree levels select the system (0 for one system)
+  uint32_t nsys = 1;
+  int vlev = 0;
+  std::vector<uint64_t> sys_off;
+  Buf d_sysoff;
   // state
   bool built = false, evaluated = false;
   size_t n = 0;
@@ -113,7 +119,8 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
-                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap};
+                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
+                  &d_sysoff};
     bufs.assign(std::begin(all), std::end(all));
   }
 
@@ -235,7 +242,7 @@ struct fmmh_ctx {
     ensure(bm, n * 8);
     ensure(bxyzm, n * 32);
     ensure(bbox_partial, 6 * 1024 * 8);
-    ensure(bbox, sizeof(BBox));
+    ensure(bbox, nsys * sizeof(BBox));
     ensure(flags, 64);
     ensure(chain, n * 4);
     ensure(top, n);
@@ -244,9 +251,19 @@ struct fmmh_ctx {
     ensure_scan(n);
     mark(ST_TREE);
     HCHECK(hipMemsetAsync(flags.p, 0, 64, s));
-    launch_bbox(pos, n, (double*)bbox_partial.p, (BBox*)bbox.p, s);
-    launch_keys(pos, mass, n, (const BBox*)bbox.p, (uint64_t*)keys.p, (uint32_t*)vals.p,
-                (uint32_t*)flags.p, s);
+    if (nsys == 1) {
+      launch_bbox(pos, n, (double*)bbox_partial.p, (BBox*)bbox.p, s);
+      launch_keys(pos, mass, n, (const BBox*)bbox.p, (uint64_t*)keys.p, (uint32_t*)vals.p,
+                  (uint32_t*)flags.p, s);
+    } else {
+      uint64_t* off = ensure<uint64_t>(d_sysoff, (nsys + 1) * 8);
+      uint64_t* h = (uint64_t*)host_stage((nsys + 1) * 8);
+      std::memcpy(h, sys_off.data(), (nsys + 1) * 8);
+      HCHECK(hipMemcpyAsync(off, h, (nsys + 1) * 8, hipMemcpyHostToDevice, s));
+      launch_bbox_sys(pos, off, nsys, (BBox*)bbox.p, s);
+      launch_keys_sys(pos, mass, n, off, nsys, vlev, (const BBox*)bbox.p, (uint64_t*)keys.p,
+                      (uint32_t*)vals.p, (uint32_t*)flags.p, s);
+    }
     int flip = radix_sort_u64((uint64_t*)keys.p, (uint32_t*)vals.p, (uint64_t*)keys_alt.p,
                               (uint32_t*)vals_alt.p, n, 0, 63, (uint32_t*)sort_tmp.p, s);
     keys_sorted = (uint64_t*)(flip ? keys_alt.p : keys.p);
@@ -267,7 +284,8 @@ struct fmmh_ctx {
       ensure(nchild, (size_t)cnt * 4);
       ensure(child_off, (size_t)cnt * 4);
       ensure_scan(cnt);
-      launch_split_count(bfs(), s0, cnt, keys_sorted, (uint32_t)prm.n_leaf, (uint32_t*)nchild.p, s);
+      launch_split_count(bfs(), s0, cnt, keys_sorted, (uint32_t)prm.n_leaf, vlev,
+                         (uint32_t*)nchild.p, s);
       uint32_t* tot = nullptr;
       scan_exclusive_u32((uint32_t*)nchild.p, (uint32_t*)child_off.p, cnt, (uint32_t*)scan_tmp.p, s,
                          &tot);
@@ -275,7 +293,7 @@ struct fmmh_ctx {
       if (total == 0) break;
       size_t need = (size_t)lvl_off[L + 1] + total;
       if (need > bfs_first.cap / 4) ensure_bfs(need + need / 2, true);
-      launch_split_emit(bfs(), s0, cnt, lvl_off[L + 1], keys_sorted, (uint32_t)prm.n_leaf,
+      launch_split_emit(bfs(), s0, cnt, lvl_off[L + 1], keys_sorted, (uint32_t)prm.n_leaf, vlev,
                         (uint32_t*)child_off.p, (uint32_t*)chain.p, (uint8_t*)top.p, s);
       lvl_off.push_back((uint32_t)need);
     }
@@ -303,7 +321,7 @@ struct fmmh_ctx {
     ensure(leaf_of_slot, n * 4);
     ensure_scan(nc);
     HCHECK(hipMemsetAsync(c_children.p, 0xFF, (size_t)nc * 32, s));
-    launch_scatter_cells(bfs(), nc, (uint32_t)prm.n_leaf, cells(), (uint32_t*)lvl_cells.p, s);
+    launch_scatter_cells(bfs(), nc, (uint32_t)prm.n_leaf, vlev, cells(), (uint32_t*)lvl_cells.p, s);
     for (int L = (int)max_level; L >= 0; L--)
       launch_geometry((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(),
                       bodies(), prm.center_mode, s);
@@ -555,7 +573,8 @@ struct fmmh_ctx {
     evaluate_frozen(pos, mass, acc, pot);
     if (o.drift_tol > 0.0)
       launch_drift_check((uint32_t)n, (const uint32_t*)leaf_of_slot.p, cells(), bodies(),
-                         (const BBox*)bbox.p, o.drift_tol, (uint32_t*)lf_flag.p, s);
+                         (const BBox*)bbox.p, keys_sorted, vlev, o.drift_tol,
+                         (uint32_t*)lf_flag.p, s);
     if (o.accel_cap > 0.0) launch_cap_accel(n, acc, o.accel_cap, s);
     launch_kick(n, vel, acc, st, s);
   }
@@ -671,9 +690,9 @@ struct fmmh_ctx {
     mp.pows = (const double*)pows.p;
     mp.binom = (const double*)binom.p;
     mp.fmin = (const double*)fmin.p;
-    uint2 root = make_uint2(0, 0);
-    HCHECK(hipMemcpyAsync(front_a.p, &root, 8, hipMemcpyHostToDevice, s));
-    uint32_t nf = 1;
+    // the walk starts from the self pair of every (system) root: level vlev
+    uint32_t nf = lvl_off[vlev + 1] - lvl_off[vlev];
+    launch_front_init((const uint32_t*)lvl_cells.p + lvl_off[vlev], nf, front_a.p, s);
     n_m2l = n_p2p = 0;
     walk_rounds = 0;
     Buf* fin = &front_a;
@@ -815,9 +834,35 @@ void fmmh_destroy(fmmh_ctx* ctx) { delete ctx; }
 
 const char* fmmh_last_error(const fmmh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int fmmh_set_systems(fmmh_ctx* ctx, size_t nsys, const uint64_t* offsets) {
+  if (!ctx) return FMMH_E_STATE;
+  if (nsys == 0 || nsys > FMMH_MAX_SYSTEMS)
+    return fail(ctx, FMMH_E_PARAM, "nsys must be in [1, " + std::to_string(FMMH_MAX_SYSTEMS) + "]");
+  if (!offsets || offsets[0] != 0) return fail(ctx, FMMH_E_PARAM, "offsets[0] must be 0");
+  for (size_t i = 0; i < nsys; i++)
+    if (offsets[i + 1] < offsets[i]) return fail(ctx, FMMH_E_PARAM, "offsets must not decrease");
+  if (offsets[nsys] >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
+  if (nsys > 1 && std::isfinite(ctx->prm.eps_mac))
+    return fail(ctx, FMMH_E_PARAM, "ensembles use the theta MAC (eps_mac = INFINITY)");
+  ctx->nsys = (uint32_t)nsys;
+  ctx->vlev = 0;
+  while ((1ull << (3 * ctx->vlev)) < nsys) ctx->vlev++;
+  ctx->sys_off.assign(offsets, offsets + nsys + 1);
+  ctx->built = ctx->evaluated = ctx->lists_ok = false;
+  ctx->drop_graph();
+  return FMMH_OK;
+}
+
+static int check_layout(fmmh_ctx* ctx, size_t n) {
+  if (ctx->nsys > 1 && n != ctx->sys_off[ctx->nsys])
+    return fail(ctx, FMMH_E_PARAM, "n differs from the ensemble's offsets[nsys]");
+  return FMMH_OK;
+}
+
 int fmmh_build_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass) {
   if (!ctx) return FMMH_E_STATE;
   if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  if (int rc = check_layout(ctx, n)) return rc;
   if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
   return guarded(ctx, [&] {
     ctx->build(n, d_pos, d_mass);
@@ -841,6 +886,7 @@ int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double
                       double* d_accel, double* d_pot) {
   if (!ctx) return FMMH_E_STATE;
   if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
+  if (int rc = check_layout(ctx, n)) return rc;
   if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
   return guarded(ctx, [&] {
     ctx->build(n, d_pos, d_mass);
@@ -888,6 +934,7 @@ int fmmh_leapfrog_device(fmmh_ctx* ctx, size_t n, double* d_pos, double* d_vel,
     return fail(ctx, FMMH_E_PARAM, "null argument");
   if (n == 0) return fail(ctx, FMMH_E_EMPTY, "empty input (n == 0)");
   if (n >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
+  if (int rc = check_layout(ctx, n)) return rc;
   if (!(o->dt > 0.0) || !std::isfinite(o->dt) || o->steps < 0 || o->refresh_every < 1 ||
       !(o->dt_eta >= 0.0) || !(o->dt_eps_len > 0.0) || !(o->accel_cap >= 0.0) ||
       !(o->drift_tol >= 0.0) ||

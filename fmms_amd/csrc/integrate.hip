@@ -101,14 +101,18 @@ __global__ void __launch_bounds__(IT) k_cap_accel(size_t n, double* __restrict__
 // rho + tol * max(rho, side / 2), side = the leaf's octant edge at build.
 __global__ void __launch_bounds__(IT) k_drift_check(uint32_t n, const uint32_t* __restrict__ leaf_of_slot,
                                                    DevCells c, DevBodies b,
-                                                   const BBox* __restrict__ bb, double tol,
+                                                   const BBox* __restrict__ bb,
+                                                   const uint64_t* __restrict__ keys_sorted,
+                                                   int vlev, double tol,
                                                    uint32_t* __restrict__ flag) {
   const uint32_t i = blockIdx.x * IT + threadIdx.x;
   if (i >= n) return;
   const uint32_t L = leaf_of_slot[i];
   const double dx = b.x[i] - c.cx[L], dy = b.y[i] - c.cy[L], dz = b.z[i] - c.cz[L];
   const double rho = c.rho[L];
-  const double side = ldexp(bb->side, -(int)c.level[L]);
+  // the leaf's octant edge in its own system's box (ensembles: V virtual levels)
+  const uint32_t sys = vlev ? (uint32_t)(keys_sorted[i] >> (63 - 3 * vlev)) : 0u;
+  const double side = ldexp(bb[sys].side, -((int)c.level[L] - vlev));
   const double lim = rho + tol * fmax(rho, 0.5 * side);
   if (dx * dx + dy * dy + dz * dz > lim * lim) flag[0] = 1u;
 }
@@ -147,9 +151,10 @@ void launch_cap_accel(size_t n, double* acc, double cap, hipStream_t s) {
 }
 
 void launch_drift_check(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                        const BBox* bb, double tol, uint32_t* flag, hipStream_t s) {
+                        const BBox* bb, const uint64_t* keys_sorted, int vlev, double tol,
+                        uint32_t* flag, hipStream_t s) {
   hipLaunchKernelGGL(k_drift_check, dim3((n + IT - 1) / IT), dim3(IT), 0, s, n, leaf_of_slot, c, b,
-                     bb, tol, flag);
+                     bb, keys_sorted, vlev, tol, flag);
 }
 
 void launch_fill_u64(void* p, size_t words, uint64_t v, hipStream_t s) {

@@ -56,19 +56,25 @@ struct DevBfs {
 
 // -------------------------------------------------------------- tree.hip
 void launch_bbox(const double* pos, size_t n, double* partial, BBox* out, hipStream_t s);
+// Ensembles of nsys systems (body ranges off[s]..off[s+1], device array).
+void launch_bbox_sys(const double* pos, const uint64_t* off, uint32_t nsys, BBox* out,
+                     hipStream_t s);
+void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,
+                     uint32_t nsys, int vlev, const BBox* bbs, uint64_t* keys, uint32_t* vals,
+                     uint32_t* flags, hipStream_t s);
 void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb,
                  uint64_t* keys, uint32_t* vals, uint32_t* flags, hipStream_t s);
 void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
                    DevBodies b, hipStream_t s);
 void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipStream_t s);
 void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
-                        uint32_t n_leaf, uint32_t* nchild, hipStream_t s);
+                        uint32_t n_leaf, int vlev, uint32_t* nchild, hipStream_t s);
 void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
-                       uint32_t n_leaf, const uint32_t* child_off, uint32_t* chain,
+                       uint32_t n_leaf, int vlev, const uint32_t* child_off, uint32_t* chain,
                        uint8_t* top, hipStream_t s);
 void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8_t* top,
                     hipStream_t s);
-void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, DevCells c,
+void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, int vlev, DevCells c,
                           uint32_t* lvl_cells, hipStream_t s);
 void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
                      int center_mode, hipStream_t s);
@@ -105,6 +111,7 @@ struct MacParams {
   const double* fmin;
 };
 size_t walk_rec_bytes();
+void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s);
 void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s);
 void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
                        uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
@@ -189,7 +196,8 @@ void launch_kick(size_t n, double* vel, const double* acc, const double* st, hip
 void launch_cap_accel(size_t n, double* acc, double cap, hipStream_t s);
 void launch_fill_u64(void* p, size_t words, uint64_t v, hipStream_t s);
 void launch_drift_check(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                        const BBox* bb, double tol, uint32_t* flag, hipStream_t s);
+                        const BBox* bb, const uint64_t* keys_sorted, int vlev, double tol,
+                        uint32_t* flag, hipStream_t s);
 
 // ------------------------------------------------------------ direct.hip
 void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,

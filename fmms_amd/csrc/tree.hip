@@ -95,6 +95,54 @@ void launch_bbox(const double* pos, size_t n, double* partial, BBox* out, hipStr
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(TB), 0, s, partial, nb, out);
 }
 
+// Ensembles: one block per system reduces its body range [off[s], off[s+1]).
+__global__ void __launch_bounds__(TB) k_bbox_sys(const double* __restrict__ pos,
+                                                const uint64_t* __restrict__ off,
+                                                BBox* __restrict__ out) {
+  __shared__ double sh[6][TB];
+  const uint32_t sys = blockIdx.x;
+  const size_t b = off[sys], e = off[sys + 1];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = b + threadIdx.x; i < e; i += TB)
+    for (int d = 0; d < 3; d++) {
+      double v = pos[3 * i + d];
+      lo[d] = fmin(lo[d], v);
+      hi[d] = fmax(hi[d], v);
+    }
+  for (int d = 0; d < 3; d++) {
+    sh[d][threadIdx.x] = lo[d];
+    sh[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int st = TB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int d = 0; d < 3; d++) {
+        sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + st]);
+        sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < 3; d++) {
+    lo[d] = sh[d][0];
+    hi[d] = sh[3 + d][0];
+  }
+  double side = fmax(fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]), 2.2250738585072014e-308);
+  BBox o;
+  for (int d = 0; d < 3; d++) {
+    o.lo[d] = lo[d];
+    o.hi[d] = hi[d];
+  }
+  o.side = side;
+  o.inv_side = 1.0 / side;
+  out[sys] = o;
+}
+
+void launch_bbox_sys(const double* pos, const uint64_t* off, uint32_t nsys, BBox* out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_sys, dim3(nsys), dim3(TB), 0, s, pos, off, out);
+}
+
 // Spread the low 21 bits of x to every third bit (tree.rs:28-35).
 __device__ __forceinline__ uint64_t dilate21(uint64_t x) {
   x &= (1ull << BITS) - 1;
@@ -132,6 +180,45 @@ __global__ void __launch_bounds__(TB) k_keys(const double* __restrict__ pos,
   keys[i] = key;
   vals[i] = (uint32_t)i;
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
+}
+
+// Ensemble keys: the body's Morton key within its own system's box (the
+// standalone key) with its low 3V bits dropped, under a 3V-bit system prefix
+// (V virtual levels, nsys <= 8^V). Below level V the forest splits exactly
+// like each system's standalone tree.
+__global__ void __launch_bounds__(TB) k_keys_sys(const double* __restrict__ pos,
+                                                const double* __restrict__ mass, size_t n,
+                                                const uint64_t* __restrict__ off, uint32_t nsys,
+                                                int vlev, const BBox* __restrict__ bbs,
+                                                uint64_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals,
+                                                uint32_t* __restrict__ flags) {
+  size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = nsys;  // system s with off[s] <= i < off[s+1]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const BBox* bb = bbs + lo;
+  const double lo0 = bb->lo[0], inv = bb->inv_side;
+  double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2], m = mass[i];
+  uint64_t key = dilate21(quantise(z, lo0, inv));
+  key |= dilate21(quantise(y, lo0, inv)) << 1;
+  key |= dilate21(quantise(x, lo0, inv)) << 2;
+  keys[i] = ((uint64_t)lo << (63 - 3 * vlev)) | (key >> (3 * vlev));
+  vals[i] = (uint32_t)i;
+  if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
+}
+
+void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,
+                     uint32_t nsys, int vlev, const BBox* bbs, uint64_t* keys, uint32_t* vals,
+                     uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys_sys, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, n, off, nsys,
+                     vlev, bbs, keys, vals, flags);
 }
 
 void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb, uint64_t* keys,
@@ -199,20 +286,23 @@ __device__ __forceinline__ void octant_bounds(const uint64_t* __restrict__ keys,
   for (uint32_t o = 1; o < 8; o++) bnd[o] = digit_lb(keys, bnd[o - 1], f + c, shift, o);
 }
 
-__device__ __forceinline__ bool is_internal(uint32_t count, int level, uint32_t n_leaf) {
-  return !(count <= n_leaf || level >= BITS);  // tree.rs:414 leaf rule
+// tree.rs:414 leaf rule; the virtual levels of an ensemble (level < vlev,
+// cells holding several systems) always split.
+__device__ __forceinline__ bool is_internal(uint32_t count, int level, uint32_t n_leaf, int vlev) {
+  return level < vlev || !(count <= n_leaf || level >= BITS);
 }
 
 __global__ void __launch_bounds__(TB) k_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt,
                                                    const uint64_t* __restrict__ keys,
-                                                   uint32_t n_leaf, uint32_t* __restrict__ nchild) {
+                                                   uint32_t n_leaf, int vlev,
+                                                   uint32_t* __restrict__ nchild) {
   uint32_t i = blockIdx.x * TB + threadIdx.x;
   if (i >= cnt) return;
   uint32_t b = s0 + i;
   uint32_t f = bfs.first[b], c = bfs.count[b];
   int lvl = bfs.level[b];
   uint32_t k = 0;
-  if (is_internal(c, lvl, n_leaf)) {
+  if (is_internal(c, lvl, n_leaf, vlev)) {
     uint32_t bnd[9];
     octant_bounds(keys, f, c, lvl, bnd);
 #pragma unroll
@@ -222,14 +312,14 @@ __global__ void __launch_bounds__(TB) k_split_count(DevBfs bfs, uint32_t s0, uin
 }
 
 void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
-                        uint32_t n_leaf, uint32_t* nchild, hipStream_t s) {
+                        uint32_t n_leaf, int vlev, uint32_t* nchild, hipStream_t s) {
   hipLaunchKernelGGL(k_split_count, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, keys,
-                     n_leaf, nchild);
+                     n_leaf, vlev, nchild);
 }
 
 __global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt,
                                                   uint32_t e0, const uint64_t* __restrict__ keys,
-                                                  uint32_t n_leaf,
+                                                  uint32_t n_leaf, int vlev,
                                                   const uint32_t* __restrict__ child_off,
                                                   uint32_t* __restrict__ chain,
                                                   uint8_t* __restrict__ top) {
@@ -238,7 +328,7 @@ __global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint
   uint32_t b = s0 + i;
   uint32_t f = bfs.first[b], c = bfs.count[b];
   int lvl = bfs.level[b];
-  if (!is_internal(c, lvl, n_leaf)) return;
+  if (!is_internal(c, lvl, n_leaf, vlev)) return;
   uint32_t bnd[9];
   octant_bounds(keys, f, c, lvl, bnd);
   uint32_t w = e0 + child_off[i];
@@ -259,10 +349,10 @@ __global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint
 }
 
 void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
-                       uint32_t n_leaf, const uint32_t* child_off, uint32_t* chain, uint8_t* top,
-                       hipStream_t s) {
+                       uint32_t n_leaf, int vlev, const uint32_t* child_off, uint32_t* chain,
+                       uint8_t* top, hipStream_t s) {
   hipLaunchKernelGGL(k_split_emit, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, e0,
-                     keys, n_leaf, child_off, chain, top);
+                     keys, n_leaf, vlev, child_off, chain, top);
 }
 
 __global__ void __launch_bounds__(TB) k_dfs_ids(DevBfs bfs, uint32_t nc,
@@ -280,7 +370,8 @@ void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8
 }
 
 __global__ void __launch_bounds__(TB) k_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf,
-                                                     DevCells c, uint32_t* __restrict__ lvl_cells) {
+                                                     int vlev, DevCells c,
+                                                     uint32_t* __restrict__ lvl_cells) {
   uint32_t b = blockIdx.x * TB + threadIdx.x;
   if (b >= nc) return;
   uint32_t d = bfs.dfs[b];
@@ -289,7 +380,7 @@ __global__ void __launch_bounds__(TB) k_scatter_cells(DevBfs bfs, uint32_t nc, u
   c.first[d] = bfs.first[b];
   c.count[d] = cnt;
   c.level[d] = (uint8_t)lvl;
-  c.leaf[d] = is_internal(cnt, lvl, n_leaf) ? 0 : 1;
+  c.leaf[d] = is_internal(cnt, lvl, n_leaf, vlev) ? 0 : 1;
   int32_t pb = bfs.parent[b];
   if (pb >= 0) {
     uint32_t pd = bfs.dfs[pb];
@@ -301,10 +392,10 @@ __global__ void __launch_bounds__(TB) k_scatter_cells(DevBfs bfs, uint32_t nc, u
   lvl_cells[b] = d;
 }
 
-void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, DevCells c,
+void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, int vlev, DevCells c,
                           uint32_t* lvl_cells, hipStream_t s) {
-  hipLaunchKernelGGL(k_scatter_cells, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, bfs, nc, n_leaf, c,
-                     lvl_cells);
+  hipLaunchKernelGGL(k_scatter_cells, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, bfs, nc, n_leaf,
+                     vlev, c, lvl_cells);
 }
 
 // ---------------------------------------------------------------- geometry

@@ -38,6 +38,19 @@ __global__ void __launch_bounds__(256) k_pack_walk(uint32_t nc, DevCells c,
 
 size_t walk_rec_bytes() { return sizeof(WalkRec); }
 
+// Initial walk frontier: the self pairs (c, c) of the given cells (the root,
+// or every system root of an ensemble).
+__global__ void __launch_bounds__(256) k_front_init(const uint32_t* __restrict__ cells,
+                                                   uint32_t cnt, uint2* __restrict__ front) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < cnt) front[i] = make_uint2(cells[i], cells[i]);
+}
+
+void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s) {
+  hipLaunchKernelGGL(k_front_init, dim3((cnt + 255) / 256), dim3(256), 0, s, cells, cnt,
+                     (uint2*)front);
+}
+
 void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s) {
   hipLaunchKernelGGL(k_pack_walk, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, (WalkRec*)rec);
 }

@@ -96,6 +96,20 @@ int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double
  * output pointer may be NULL. children [nc][8], -1 where absent; cen [nc][3]. */
 size_t fmmh_num_cells(const fmmh_ctx* ctx);
 
+/* Batched ensembles (no reference counterpart: the reference evaluates one
+ * system per Fmm). After this call the context treats its n = offsets[nsys]
+ * bodies as nsys independent systems with bodies offsets[s]..offsets[s+1]-1
+ * (host array, offsets[0] = 0, non-decreasing, nsys <= FMMH_MAX_SYSTEMS): one
+ * forest whose top V = ceil(log8 nsys) levels select the system and whose
+ * subtrees are the systems' own octrees (each keyed in its own bounding box,
+ * at 21 - V bits per axis), walked from the systems' roots so no interaction
+ * crosses systems; every stage runs once for the whole ensemble. Every
+ * evaluation and leapfrog entry point then works on the ensemble (the
+ * leapfrog's dt clamp uses the ensemble-wide maximum acceleration). Theta
+ * MAC only (eps_mac = INFINITY). nsys = 1 restores the single-system mode. */
+#define FMMH_MAX_SYSTEMS 4096
+int fmmh_set_systems(fmmh_ctx* ctx, size_t nsys, const uint64_t* offsets);
+
 /* Temporal reuse — the frozen step of crates/visual/src/gpufmm.rs:2710-2741
  * (refresh model gpufmm.rs:9-20): evaluate on the tree and interaction lists
  * of this context's last full evaluation. Bodies are re-gathered into the
