@@ -615,7 +615,13 @@ struct fmmh_ctx {
       drifted = false;
       lf_stats[1]++;
     };
-    if (!(o.flags & FMMH_LF_ACC_VALID) || !lists_ok || n != n_) full();
+    if (!(o.flags & FMMH_LF_ACC_VALID)) {
+      full();
+    } else if (!lists_ok || n != n_) {
+      // accelerations given, no current tree: the first step rebuilds
+      n = n_;
+      tree_evals = (uint32_t)o.refresh_every;
+    }
     const bool use_graph = (o.flags & FMMH_LF_GRAPH) != 0;
     const GraphKey key{pos, vel, mass, acc, pot, o.dt, o.dt_eta, o.dt_eps_len, o.accel_cap,
                        o.drift_tol};

@@ -36,6 +36,14 @@ def _check(rc, what):
         raise RuntimeError(f"{what} failed: {runtime().hipGetErrorString(rc).decode()}")
 
 
+def copy_d2d(dst_ptr, src_ptr, nbytes):
+    """Device-to-device copy of nbytes between raw device addresses (ordered
+    on the null stream; synchronize() before handing the data to a library
+    stream)."""
+    if nbytes:
+        _check(runtime().hipMemcpy(dst_ptr, src_ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def device_count():
     n = ct.c_int(0)
     rc = runtime().hipGetDeviceCount(ct.byref(n))

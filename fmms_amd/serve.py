@@ -1,0 +1,247 @@
+"""Serving loop: continuous batching of independent N-body systems on one GPU.
+
+Requests (a new system plus a number of leapfrog steps) arrive over time.
+The server admits them under an HBM budget, computes each newcomer's
+initial forces on its own, and from the next step boundary advances it
+together with every other active system: all active systems form one
+ensemble (fmmh_set_systems) whose state (positions, velocities, masses,
+accelerations, potentials) stays packed in device memory, so one batched
+kick-drift-kick step (fmmh_leapfrog_device) advances them all with one
+launch per stage. Finished systems leave at the next boundary, their final
+state copied out. Each system's trajectory is the one it would have alone:
+the forest evaluates every system exactly as its standalone tree would.
+
+The reference has no serving layer (crates/visual/src/sim.rs steps one
+system per Sim); this is the batched form of its Sim::step loop
+(sim.rs:539-594) for many systems.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Fmm, FmmError, hip, ic_disk, ic_hernquist, ic_plummer
+
+# Device bytes per body of one evaluation context at p = 8 (bodies, keys,
+# cells, expansions and lists), measured on the 1M-body Plummer workload and
+# rounded up; used for admission under the HBM budget.
+BYTES_PER_BODY = 1600
+
+
+@dataclass
+class Request:
+    """A system to evolve: initial state (host arrays) and a step count."""
+    rid: int
+    pos: np.ndarray
+    vel: np.ndarray
+    mass: np.ndarray
+    steps: int
+    arrival: float = 0.0  # seconds since the start of the run
+    # filled by the server
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps_done: int = 0
+    final: tuple = field(default=None, repr=False)
+
+    @property
+    def n(self):
+        return self.mass.shape[0]
+
+
+def make_request(rid, model, n, steps, seed, arrival=0.0):
+    """A Plummer, Hernquist or exponential-disk system (total mass 1)."""
+    if model == "plummer":
+        s = ic_plummer(n, rmax=20.0, seed=seed)
+    elif model == "hernquist":
+        s = ic_hernquist(n, a=1.0, rmax=100.0, seed=seed)
+    elif model == "disk":
+        s = ic_disk(n, rd=1.0, zd=0.05, rmax=10.0, m_central=0.1, seed=seed)
+    else:
+        raise ValueError(model)
+    return Request(rid, s.pos, s.vel, s.mass, steps, arrival)
+
+
+class _Slot:
+    """One admitted system: its device state lives either in its own arrays
+    (`arrays`, while waiting to join) or in the packed ensemble at `offset`."""
+
+    def __init__(self, req, arrays):
+        self.req = req
+        self.arrays = arrays  # dict name -> DeviceArray (pos, vel, mass, acc, pot)
+        self.offset = None    # position in the packed arrays once joined
+
+
+class Server:
+    """Continuous-batching leapfrog server (one GPU, one ensemble context)."""
+
+    FIELDS = (("pos", 3), ("vel", 3), ("mass", 1), ("acc", 3), ("pot", 1))
+
+    def __init__(self, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4,
+                 hbm_budget=200e9):
+        self.kw = dict(p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening)
+        self.dt = dt
+        self.hbm_budget = float(hbm_budget)
+        self.ctx = Fmm(None, None, **self.kw)
+        self.admit_ctx = Fmm(None, None, **self.kw)
+        self.active = []      # admitted systems, in packing order
+        self.queue = []       # arrived, not yet admitted
+        self.packed = None    # dict name -> DeviceArray over all active systems
+        self.dirty = True     # membership changed since the last packing
+        self.step_ms = []     # wall time of each batched step
+        self.step_bodies = []
+
+    # ---------------------------------------------------------- admission
+    def _bytes(self, n):
+        return BYTES_PER_BODY * n
+
+    def _in_use(self):
+        return sum(self._bytes(s.req.n) for s in self.active)
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _admit(self, now):
+        kept = []
+        for req in self.queue:
+            if self._in_use() + self._bytes(req.n) > self.hbm_budget:
+                kept.append(req)
+                continue
+            n = req.n
+            arr = {"pos": hip.DeviceArray.from_host(np.ascontiguousarray(req.pos)),
+                   "vel": hip.DeviceArray.from_host(np.ascontiguousarray(req.vel)),
+                   "mass": hip.DeviceArray.from_host(np.ascontiguousarray(req.mass)),
+                   "acc": hip.DeviceArray((n, 3)), "pot": hip.DeviceArray((n,))}
+            # a(x0) on its own (the same forces it gets inside an ensemble)
+            self.admit_ctx.set_systems([n])
+            self.admit_ctx.accel_device(n, arr["pos"].ptr, arr["mass"].ptr, arr["acc"].ptr,
+                                        arr["pot"].ptr)
+            req.admitted = now
+            self.active.append(_Slot(req, arr))
+            self.dirty = True
+        self.queue = kept
+
+    # ------------------------------------------------------------ packing
+    def _pack(self):
+        """New packed arrays for the current active set: joined systems move
+        from the old packed arrays, newcomers from their own arrays."""
+        total = sum(s.req.n for s in self.active)
+        new = {name: hip.DeviceArray((total, w) if w > 1 else (total,))
+               for name, w in self.FIELDS}
+        off = 0
+        for s in self.active:
+            for name, w in self.FIELDS:
+                if s.offset is None:
+                    src = s.arrays[name].ptr
+                else:
+                    src = self.packed[name].ptr + s.offset * w * 8
+                hip.copy_d2d(new[name].ptr + off * w * 8, src, s.req.n * w * 8)
+            off += s.req.n
+        hip.synchronize()
+        off = 0
+        for s in self.active:
+            if s.offset is None:
+                for a in s.arrays.values():
+                    a.free()
+                s.arrays = None
+            s.offset = off
+            off += s.req.n
+        if self.packed is not None:
+            for a in self.packed.values():
+                a.free()
+        self.packed = new
+        self.ctx.set_systems([s.req.n for s in self.active])
+        self.dirty = False
+
+    def _retire(self, slot):
+        """Final state of a finished system (host arrays) from the packed set."""
+        out = []
+        for name, w in self.FIELDS:
+            if name == "mass":
+                continue
+            shape = (slot.req.n, w) if w > 1 else (slot.req.n,)
+            a = hip.DeviceArray(shape)
+            hip.copy_d2d(a.ptr, self.packed[name].ptr + slot.offset * w * 8, slot.req.n * w * 8)
+            out.append(a)
+        hip.synchronize()
+        res = tuple(a.download() for a in out)
+        for a in out:
+            a.free()
+        return res
+
+    # --------------------------------------------------------------- step
+    def step(self, now):
+        """One step boundary: admit arrivals (initial forces, packing), one
+        batched KDK step of every active system, retire finished systems.
+        Returns the number of bodies stepped."""
+        t0 = time.perf_counter()
+        self._admit(now)
+        if not self.active:
+            return 0
+        if self.dirty:
+            self._pack()
+        total = sum(s.req.n for s in self.active)
+        p = self.packed
+        self.ctx.leapfrog_device(total, p["pos"].ptr, p["vel"].ptr, p["mass"].ptr, p["acc"].ptr,
+                                 p["pot"].ptr, self.dt, 1, refresh_every=1, acc_valid=True)
+        done = []
+        for s in self.active:
+            s.req.steps_done += 1
+            if s.req.steps_done >= s.req.steps:
+                done.append(s)
+        for s in done:
+            s.req.final = self._retire(s)
+            s.req.finished = now
+        if done:
+            self.active = [s for s in self.active if s not in done]
+            self.dirty = True
+        self.step_ms.append(1e3 * (time.perf_counter() - t0))
+        self.step_bodies.append(total)
+        return total
+
+    def run(self, requests, max_seconds=None):
+        """Serve requests (sorted by arrival, seconds since start) until all
+        are finished; arrivals are admitted at the first step boundary after
+        their arrival time."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        t_start = time.perf_counter()
+        i = 0
+        while i < len(pending) or self.queue or self.active:
+            now = time.perf_counter() - t_start
+            if max_seconds is not None and now > max_seconds:
+                break
+            while i < len(pending) and pending[i].arrival <= now:
+                self.submit(pending[i])
+                i += 1
+            if not self.queue and not self.active:
+                time.sleep(max(0.0, min(pending[i].arrival - now, 0.01)))
+                continue
+            self.step(now)
+        return time.perf_counter() - t_start
+
+    def stats(self):
+        ms = np.array(self.step_ms)
+        bodies = np.array(self.step_bodies, dtype=float)
+        if ms.size == 0:
+            return {}
+        return {"steps": int(ms.size), "p50_ms": float(np.percentile(ms, 50)),
+                "p99_ms": float(np.percentile(ms, 99)),
+                "body_forces_per_s": float(bodies.sum() / (ms.sum() * 1e-3)),
+                "mean_batch_bodies": float(bodies.mean())}
+
+
+def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4):
+    """The same system stepped on its own (the reference behaviour: one Sim)."""
+    from . import Integrator
+
+    f = Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening)
+    it = Integrator(f, req.pos, req.vel, req.mass)
+    it.step(dt, steps=0)
+    t0 = time.perf_counter()
+    for _ in range(req.steps):
+        it.step(dt, steps=1, refresh_every=1)
+    el = time.perf_counter() - t0
+    return it.state(), el
+
+
+__all__ = ["Request", "Server", "make_request", "run_alone", "FmmError"]

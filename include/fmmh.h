@@ -137,8 +137,8 @@ int fmmh_evaluate_frozen_device(fmmh_ctx* ctx, size_t n, const double* d_pos,
  * gpufmm.rs:14-20, bounds the same hazard by step count only). With FMMH_LF_GRAPH the frozen step (dt, kicks, drift and
  * the whole force evaluation) is captured once per tree into a hipGraph and
  * replayed. FMMH_LF_ACC_VALID: accel/pot already hold a(x) of the entry
- * positions from this context's last evaluation (else they are evaluated
- * first). pos/vel/accel [n][3], pot [n], mass [n] are device pointers in id
+ * positions (else they are evaluated first); when this context has no tree
+ * for them, the first step rebuilds. pos/vel/accel [n][3], pot [n], mass [n] are device pointers in id
  * order; on return accel/pot belong to the final positions and
  * *time_advanced (host, may be NULL) is the sum of the steps' dt. */
 #define FMMH_LF_ACC_VALID 1u
