@@ -248,6 +248,149 @@ def bench_halos(fa, hip, args):
     print(json.dumps(out), flush=True)
 
 
+def bench_ensemble(fa, hip, args):
+    """Batched ensemble: --systems systems, N log-uniform in [2048, 524288],
+    half Plummer (r <= 20), half exponential disk (scale length 1, height
+    0.05, 10 % central mass, Keplerian + self-gravity circular velocities);
+    softening 1e-3, p = 8, theta = 0.5. A step is one force evaluation of the
+    whole ensemble (one launch per stage); beside it, the same systems
+    evaluated one after another (one context per system)."""
+    rng = np.random.default_rng(7)
+    k = args.systems
+    sizes = np.exp(rng.uniform(np.log(2048), np.log(524288), k)).astype(int)
+    soft = 1e-3
+    log(f"[bench] generating {k} systems, {int(sizes.sum())} bodies")
+    systems = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            systems.append(fa.ic_plummer(int(n), rmax=20.0, seed=1000 + i))
+        else:
+            systems.append(fa.ic_disk(int(n), rd=1.0, zd=0.05, rmax=10.0, m_central=0.1,
+                                      seed=2000 + i))
+    pos = np.concatenate([s.pos for s in systems])
+    mass = np.concatenate([s.mass for s in systems])
+    n = len(mass)
+    d_pos, d_mass = hip.DeviceArray.from_host(pos), hip.DeviceArray.from_host(mass)
+    d_acc, d_pot = hip.DeviceArray((n, 3)), hip.DeviceArray((n,))
+    n_leaf = args.n_leaf or 32
+    ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
+                 softening=soft, timing=True)
+    off = ctx.set_systems(sizes)
+    run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(1, args.warmup))
+    log(f"[bench] timing {args.steps} ensemble evaluations")
+    elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
+    wc = ctx.work_counts()
+    acc = d_acc.download()
+    avg = {kk: float(np.mean(v)) for kk, v in stages.items()}
+    roof_stages = stage_rooflines(avg, wc, args.p)
+    dom = max(roof_stages, key=lambda kk: roof_stages[kk]["ms"])
+    roof = {"kernel": dom, "bound": "valu", "achieved": round(roof_stages[dom]["achieved"], 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(roof_stages[dom]["frac"], 4),
+            "traffic": None, "avg_launch_ms": round(roof_stages[dom]["ms"], 4)}
+    # one system at a time (the reference's model: one Fmm per system)
+    log("[bench] one-system-at-a-time pass")
+    seq = 0.0
+    rel = []
+    for i, s in enumerate(systems):
+        a, b = int(off[i]), int(off[i + 1])
+        one = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
+                     softening=soft)
+        dp, dm = hip.DeviceArray.from_host(s.pos), hip.DeviceArray.from_host(s.mass)
+        da, dq = hip.DeviceArray((s.n, 3)), hip.DeviceArray((s.n,))
+        one.accel_device(s.n, dp.ptr, dm.ptr, da.ptr, dq.ptr)
+        el, _, _ = run_steps(fa, hip, one, s.n, dp, dm, da, dq, 2)
+        seq += el / 2
+        one.close()
+        if not args.no_error:  # 16 sampled targets per system against its direct sum
+            ids = fa.sample_ids(s.n, 16, 7 + i)
+            a_ref, _, _ = fa.direct_sampled(s.pos, s.mass, ids, softening=soft)
+            rel.append(fa.rel_errors(acc[a + ids.astype(np.int64)], a_ref))
+        for x in (dp, dm, da, dq):
+            x.free()
+    err = None
+    if rel:
+        rel = np.concatenate(rel)
+        err = {"targets": int(rel.size), "rel_median": float(np.median(rel)),
+               "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max())}
+        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
+    out = {
+        "metric": METRIC, "value": n * args.steps / elapsed, "unit": "body-forces/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Plummer spheres and exponential disks)",
+        "config": {"workload": f"ensemble{k}_N{n}_p{args.p}_theta{args.theta}_soft{soft}",
+                   "systems": k, "n": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
+                   "p": args.p, "theta": args.theta, "softening": soft, "n_leaf": n_leaf,
+                   "precision": "fp64"},
+        "one_at_a_time": {"value": n / seq, "ms_per_pass": 1e3 * seq,
+                          "speedup_batched": (n * args.steps / elapsed) / (n / seq)},
+        "error": err, "roofline": roof, "stages_ms": {kk: round(v, 4) for kk, v in avg.items()},
+        "stage_roofline": roof_stages, "work": wc, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(fa, hip, args):
+    """Serving stream: --requests requests with Poisson arrivals at --rate per
+    second; each a Plummer, Hernquist or disk system (equal odds) of N
+    log-uniform in [1e3, 1e6] taking 1..100 leapfrog steps at dt = 1e-4;
+    admitted under a 200 GB HBM budget and advanced as one batched ensemble.
+    Beside it, the same requests served one system at a time."""
+    from fmms_amd import serve
+
+    rng = np.random.default_rng(11)
+    t, reqs = 0.0, []
+    models = ["plummer", "hernquist", "disk"]
+    for i in range(args.requests):
+        t += rng.exponential(1.0 / args.rate)
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(1e6))))
+        reqs.append(serve.make_request(i, models[rng.integers(3)], n, int(rng.integers(1, 101)),
+                                       seed=500 + i, arrival=t))
+    total_steps = sum(r.n * r.steps for r in reqs)
+    log(f"[bench] {len(reqs)} requests, {sum(r.n for r in reqs)} bodies, "
+        f"{total_steps} body-steps, arrivals over {t:.1f} s")
+    n_leaf = args.n_leaf or 32
+    srv = serve.Server(p=args.p, theta=args.theta, softening=1e-3, n_leaf=n_leaf, dt=1e-4,
+                       hbm_budget=200e9)
+    wall = srv.run(reqs)
+    st = srv.stats()
+    lat = np.array([r.finished - r.arrival for r in reqs])
+    log(f"[bench] served in {wall:.2f} s: step p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms")
+    # one system at a time, first come first served (the same requests)
+    log("[bench] one-system-at-a-time pass")
+    alone = 0.0
+    alone_ms = []
+    for r in reqs:
+        _, el = serve.run_alone(r, p=args.p, theta=args.theta, softening=1e-3, n_leaf=n_leaf,
+                                dt=1e-4)
+        alone += el
+        alone_ms.append(1e3 * el / r.steps)
+    out = {
+        "metric": METRIC, "value": st["body_forces_per_s"], "unit": "body-forces/s",
+        "n_gpus": 1, "steps": st["steps"], "warmup": 0,
+        "ms_per_step": float(np.mean(srv.step_ms)),
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Plummer, Hernquist and disk systems, Poisson arrivals)",
+        "config": {"workload": f"serve_{len(reqs)}req_rate{args.rate}_p{args.p}_theta{args.theta}",
+                   "requests": len(reqs), "rate_per_s": args.rate, "p": args.p,
+                   "theta": args.theta, "softening": 1e-3, "n_leaf": n_leaf, "dt": 1e-4,
+                   "hbm_budget_bytes": 200e9, "precision": "fp64"},
+        "serving": {"wall_s": wall, "body_steps": total_steps,
+                    "wall_body_forces_per_s": total_steps / wall,
+                    "request_latency_p50_s": float(np.percentile(lat, 50)),
+                    "request_latency_p99_s": float(np.percentile(lat, 99)),
+                    "mean_batch_bodies": st["mean_batch_bodies"]},
+        "one_at_a_time": {"seconds": alone, "body_forces_per_s": total_steps / alone,
+                          "step_p50_ms": float(np.percentile(alone_ms, 50)),
+                          "step_p99_ms": float(np.percentile(alone_ms, 99))},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,7 +408,12 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--workload", choices=["plummer", "halos"], default="plummer")
+    ap.add_argument("--workload", choices=["plummer", "halos", "ensemble", "serve"],
+                    default="plummer")
+    ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
+    ap.add_argument("--rate", type=float, default=20.0,
+                    help="serve: Poisson arrival rate (requests per second)")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--refresh", type=int, default=10,
                     help="halos: rebuild the tree and lists every this many steps")
     ap.add_argument("--dt", type=float, default=1e-4)
@@ -286,6 +434,10 @@ def main():
 
     if args.workload == "halos":
         return bench_halos(fa, hip, args)
+    if args.workload == "ensemble":
+        return bench_ensemble(fa, hip, args)
+    if args.workload == "serve":
+        return bench_serve(fa, hip, args)
 
     n = args.n
     log(f"[bench] generating Plummer N={n} (r <= 20)")
