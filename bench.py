@@ -70,7 +70,7 @@ def run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, k):
     return time.perf_counter() - t0, lat, stages
 
 
-def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot):
+def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot, precision="fp64"):
     """Fastest (p, n_leaf) over p in --p-grid and n_leaf in 16..128 whose
     relative force error meets the metric's bar (99th percentile <= --max-rel-err
     against the FP64 direct sum on the sampled targets)."""
@@ -82,7 +82,7 @@ def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot):
     for p in grid_p:
         for nl in leaves:
             ctx = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=nl, theta_max=args.theta,
-                         softening=args.softening)
+                         softening=args.softening, precision=precision)
             ctx.accel_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
             el, _, _ = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, 2)
             ctx.close()
@@ -91,7 +91,7 @@ def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot):
             ok = p99 <= args.max_rel_err
             tried.append({"p": p, "n_leaf": nl, "ms": round(1e3 * el / 2, 3), "rel_p99": p99,
                           "valid": ok})
-            log(f"[autotune] p={p} n_leaf={nl}: {1e3 * el / 2:.2f} ms/step, rel p99 {p99:.2e}"
+            log(f"[autotune] {precision} p={p} n_leaf={nl}: {1e3 * el / 2:.2f} ms/step, rel p99 {p99:.2e}"
                 f"{'' if ok else ' (above the bar)'}")
             if ok and (best is None or el < best[2]):
                 best = (p, nl, el)
@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--forms", default="fp64,fp32bulk",
+                    help="plummer: precision forms measured (the fastest valid one is reported)")
     ap.add_argument("--workload", choices=["plummer", "halos", "ensemble", "serve"],
                     default="plummer")
     ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
@@ -446,70 +448,72 @@ def main():
     d_mass = hip.DeviceArray.from_host(mass)
     d_acc = hip.DeviceArray((n, 3))
     d_pot = hip.DeviceArray((n,))
-
-    tuned = None
-    if args.n_leaf:
-        n_leaf = args.n_leaf
-    else:
-        args.p, n_leaf, tuned = autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot)
-    log(f"[bench] p={args.p} n_leaf={n_leaf}")
-    ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
-                 softening=args.softening, timing=True)
-    run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(args.warmup, 1))
-    log(f"[bench] timing {args.steps} steps")
-    elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
-    wc = ctx.work_counts()
-    acc = d_acc.download()
-
-    ms_step = 1e3 * elapsed / args.steps
-    avg = {k: float(np.mean(v)) for k, v in stages.items()}
-    # algorithmic work per launch
-    m2l_dir = 2 * wc["m2l_pairs"]
-    m2l_flops = 8.0 * t_full(args.p) * m2l_dir
-    p2p_flops = 22.0 * wc["p2p_interactions"]
-    kernels = {
-        "m2l": ("valu", m2l_flops, "TFLOP/s"),
-        "p2p": ("valu", p2p_flops, "TFLOP/s"),
-    }
-    stage_roof = {}
-    for k, (bound, work, unit) in kernels.items():
-        if avg.get(k, 0) > 0:
-            ach = work / (avg[k] * 1e-3) / 1e12
-            stage_roof[k] = {"ms": avg[k], "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                             "frac": ach / FP64_PEAK_TFLOPS}
-    dom = max(stage_roof, key=lambda k: stage_roof[k]["ms"])
-    roof = {"kernel": dom, "bound": kernels[dom][0], "achieved": round(stage_roof[dom]["achieved"], 3),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(stage_roof[dom]["frac"], 4), "traffic": None,
-            "work_per_launch": kernels[dom][1], "avg_launch_ms": round(stage_roof[dom]["ms"], 4)}
-
-    # HBM bytes per launch of the dominant kernel, measured by rocprofv3
-    # FETCH_SIZE/WRITE_SIZE runs of this workload (profiles/), when present.
-    tf = os.path.join(ROOT, "profiles", "traffic_plummer1M_p8_theta0.5_fp64.json")
-    if os.path.exists(tf):
-        t = json.load(open(tf))
-        if (t.get("n_leaf") == n_leaf and t.get("workload") == f"plummer_N{n}_p{args.p}_theta{args.theta}"
-                f"_soft{args.softening}_fp64" and dom in t.get("bytes_per_launch", {})):
-            roof["traffic"] = t["bytes_per_launch"][dom]
-            roof["traffic_source"] = t["source"]
-
-    err = None
+    ids = fa.sample_ids(n, args.samples, 7)
+    a_ref = f_ref = None
     if not args.no_error:
-        ids = fa.sample_ids(n, args.samples, 7)
         log(f"[bench] FP64 direct sum on {len(ids)} sampled targets")
         a_ref, _, f_ref = fa.direct_sampled(pos, mass, ids, softening=args.softening)
-        rel = fa.rel_errors(acc[ids], a_ref)
-        scaled = np.linalg.norm(acc[ids] - a_ref, axis=1) / f_ref
-        err = {"targets": int(len(ids)), "rel_median": float(np.median(rel)),
-               "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max()),
-               "scaled_rms": float(np.sqrt(np.mean(scaled ** 2))), "scaled_max": float(scaled.max())}
-        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
+
+    def measure(precision):
+        """Autotune (unless --n-leaf), then time --steps evaluations of one form."""
+        tuned, p = None, args.p
+        if args.n_leaf:
+            n_leaf = args.n_leaf
+        else:
+            p, n_leaf, tuned = autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot,
+                                        precision)
+        log(f"[bench] {precision}: p={p} n_leaf={n_leaf}")
+        ctx = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
+                     softening=args.softening, timing=True, precision=precision)
+        run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(args.warmup, 1))
+        log(f"[bench] {precision}: timing {args.steps} steps")
+        elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
+        wc = ctx.work_counts()
+        acc = d_acc.download()
+        ctx.close()
+        err = None
+        if a_ref is not None:
+            rel = fa.rel_errors(acc[ids], a_ref)
+            scaled = np.linalg.norm(acc[ids] - a_ref, axis=1) / f_ref
+            err = {"targets": int(len(ids)), "rel_median": float(np.median(rel)),
+                   "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max()),
+                   "scaled_rms": float(np.sqrt(np.mean(scaled ** 2))),
+                   "scaled_max": float(scaled.max())}
+            log(f"[bench] {precision}: rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
+        return {"precision": precision, "p": p, "n_leaf": n_leaf, "elapsed": elapsed, "lat": lat,
+                "stages": {k: float(np.mean(v)) for k, v in stages.items()}, "wc": wc,
+                "acc": acc, "tuned": tuned, "err": err,
+                "value": n * args.steps / elapsed}
+
+    forms = [measure(f) for f in args.forms.split(",")]
+    valid = [f for f in forms if f["err"] is None or f["err"]["rel_p99"] <= args.max_rel_err]
+    best = max(valid or forms, key=lambda f: f["value"])
+    fp64 = next((f for f in forms if f["precision"] == "fp64"), None)
+    args.p = best["p"]
+    n_leaf, elapsed, lat, avg, wc = best["n_leaf"], best["elapsed"], best["lat"], best["stages"], best["wc"]
+    ms_step = 1e3 * elapsed / args.steps
+    stage_roof = stage_rooflines(avg, wc, args.p)
+    dom = max(stage_roof, key=lambda k: stage_roof[k]["ms"])
+    work = {"m2l": 8.0 * t_full(args.p) * 2 * wc["m2l_pairs"], "p2p": 22.0 * wc["p2p_interactions"]}
+    roof = {"kernel": dom, "bound": "valu", "achieved": round(stage_roof[dom]["achieved"], 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(stage_roof[dom]["frac"], 4), "traffic": None,
+            "work_per_launch": work[dom], "avg_launch_ms": round(stage_roof[dom]["ms"], 4)}
+    # HBM bytes per launch of the dominant kernel, measured by rocprofv3
+    # FETCH_SIZE/WRITE_SIZE runs of this workload (profiles/), when present.
+    tag = f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_{best['precision']}"
+    tf = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf))
+        if t.get("n_leaf") == n_leaf and dom in t.get("bytes_per_launch", {}):
+            roof["traffic"] = t["bytes_per_launch"][dom]
+            roof["traffic_source"] = t["source"]
 
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, threaded)")
         acc_cpu, cpu = cpu_baseline(pos, mass, args, n_leaf)
-        d = np.linalg.norm(acc - acc_cpu, axis=1) / np.linalg.norm(acc_cpu, axis=1)
+        d = np.linalg.norm(best["acc"] - acc_cpu, axis=1) / np.linalg.norm(acc_cpu, axis=1)
         cpu["max_rel_diff_vs_gpu"] = float(d.max())
 
     if args.calibrate_copy:
@@ -521,9 +525,11 @@ def main():
         src.free()
         dst.free()
 
+    form_names = {"fp64": "plain FP64 GPU drop-in",
+                  "fp32bulk": "mixed: FP32 P2P pair terms on leaf-centred positions, FP64 elsewhere"}
     out = {
         "metric": METRIC,
-        "value": n * args.steps / elapsed,
+        "value": best["value"],
         "unit": "body-forces/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -534,21 +540,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if best["precision"] == "fp64" else "f64+f32",
         "data": "synthetic (seeded Plummer sampler, radii truncated at 20)",
-        "config": {"workload": f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_fp64",
-                   "n": n, "p": args.p, "theta": args.theta, "eps_mac": "inf",
-                   "softening": args.softening, "n_leaf": n_leaf, "precision": "fp64",
-                   "form": "plain FP64 GPU drop-in",
-                   "selection": (f"fastest (p, n_leaf) over p in {{{args.p_grid}}} x n_leaf in "
-                                 f"16..128 with rel err p99 <= {args.max_rel_err:g}")
-                   if tuned else "fixed"},
-        "error": err,
+        "config": {"workload": tag, "n": n, "p": args.p, "theta": args.theta, "eps_mac": "inf",
+                   "softening": args.softening, "n_leaf": n_leaf, "precision": best["precision"],
+                   "form": form_names[best["precision"]],
+                   "selection": (f"fastest form and (p, n_leaf) over p in {{{args.p_grid}}} x "
+                                 f"n_leaf in 16..128 with rel err p99 <= {args.max_rel_err:g}")
+                   if best["tuned"] else "fixed"},
+        "error": best["err"],
+        "forms": {f["precision"]: {"value": f["value"], "ms_per_step": 1e3 * f["elapsed"] / args.steps,
+                                   "p": f["p"], "n_leaf": f["n_leaf"], "error": f["err"],
+                                   "stages_ms": {k: round(v, 4) for k, v in f["stages"].items()}}
+                  for f in forms},
+        "mixed_over_fp64": (best["value"] / fp64["value"]) if fp64 else None,
         "roofline": roof,
         "stages_ms": {k: round(v, 4) for k, v in avg.items()},
         "stage_roofline": stage_roof,
         "work": wc,
-        "autotune": tuned,
+        "autotune": {f["precision"]: f["tuned"] for f in forms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -34,6 +34,8 @@ FMMH_E_NONFINITE = 5
 FMMH_E_HIP = 6
 FMMH_E_STATE = 7
 FMMH_FLAG_TIMING = 1
+FMMH_PREC_FP64 = 0
+FMMH_PREC_FP32BULK = 1
 MAX_P = 40
 MAX_SYSTEMS = 4096
 
@@ -170,13 +172,15 @@ class Fmm:
     the near field, 0 = pure 1/r), eps = inf for the theta-only MAC."""
 
     def __init__(self, pos, mass, p, eps, n_leaf, center_mode=CenterMode.Com, theta_max=1.0,
-                 softening=0.0, timing=False, hbm_budget_bytes=0.0):
+                 softening=0.0, timing=False, hbm_budget_bytes=0.0, precision="fp64"):
         prm = _Params()
         lib().fmmh_default_params(ct.byref(prm))
         prm.p, prm.eps_mac, prm.theta_max, prm.n_leaf = int(p), float(eps), float(theta_max), int(n_leaf)
         prm.softening, prm.center_mode = float(softening), int(center_mode)
         prm.flags = FMMH_FLAG_TIMING if timing else 0
         prm.hbm_budget_bytes = float(hbm_budget_bytes)
+        prm.precision = {"fp64": FMMH_PREC_FP64, "fp32bulk": FMMH_PREC_FP32BULK}[precision]
+        self.precision = precision
         self.p, self.eps, self.n_leaf = int(p), float(eps), int(n_leaf)
         self.theta_max, self.softening, self.center_mode = float(theta_max), float(softening), int(center_mode)
         h = _vp()

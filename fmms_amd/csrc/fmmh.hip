@@ -498,8 +498,12 @@ struct fmmh_ctx {
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
     mark(ST_P2P);
-    launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
-               b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
+    if (prm.precision == FMMH_PREC_FP32BULK)
+      launch_p2p_f32((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
+                     b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
+    else
+      launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
+                 b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
     mark(ST_COUNT);
   }
 
@@ -763,8 +767,8 @@ int check_params(const fmmh_params* p, std::string* why) {
     *why = "center_mode must be FMMH_CENTER_COM or FMMH_CENTER_SPHERE";
     return FMMH_E_PARAM;
   }
-  if (p->precision != FMMH_PREC_FP64) {
-    *why = "only FMMH_PREC_FP64 is available";
+  if (p->precision != FMMH_PREC_FP64 && p->precision != FMMH_PREC_FP32BULK) {
+    *why = "precision must be FMMH_PREC_FP64 or FMMH_PREC_FP32BULK";
     return FMMH_E_PARAM;
   }
   return FMMH_OK;

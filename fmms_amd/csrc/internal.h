@@ -179,6 +179,10 @@ void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr
                 const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, const double4* body,
                 uint2* prange, double soft2, const double4* far, const uint32_t* order,
                 double* accel, double* pot, hipStream_t s);
+void launch_p2p_f32(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                    const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c,
+                    const double4* body, uint2* prange, double soft2, const double4* far,
+                    const uint32_t* order, double* accel, double* pot, hipStream_t s);
 // Directed body-body interactions of a P2P pair list (self pairs c(c-1));
 // out[0] = total, out needs 1 + p2p_count_scratch(np) words.
 size_t p2p_count_scratch(uint32_t np);

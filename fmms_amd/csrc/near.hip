@@ -155,6 +155,129 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
   }
 }
 
+// ----------------------------------------------------------- FP32 bulk form
+// Same traversal as k_p2p; sources are staged as float4 positions relative
+// to the target leaf's centre (the subtraction in FP64), the pairwise terms
+// run in FP32 with the hardware rsqrt, and each tile's FP32 partial sums are
+// added into FP64 accumulators.
+template <bool SELF>
+__device__ __forceinline__ void p2p_tile32(const float4* __restrict__ tile, uint32_t sub,
+                                           uint32_t K, float xi, float yi, float zi, float soft2,
+                                           int self_j, float& ax, float& ay, float& az,
+                                           float& ph) {
+#pragma unroll 4
+  for (uint32_t j = sub; j < 64; j += K) {
+    const float4 sj = tile[j];
+    const float dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
+    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+    float mj = sj.w;
+    if (SELF) {
+      const bool self = (int)j == self_j;
+      r2 = self ? 1.0f : r2;
+      mj = self ? 0.0f : mj;
+    }
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float minv = mj * inv;
+    const float s3 = minv * inv * inv;
+    ax = fmaf(s3, dx, ax);
+    ay = fmaf(s3, dy, ay);
+    az = fmaf(s3, dz, az);
+    ph += minv;
+  }
+}
+
+__device__ __forceinline__ float4 rel32(double4 v, double cx, double cy, double cz) {
+  return make_float4((float)(v.x - cx), (float)(v.y - cy), (float)(v.z - cz), (float)v.w);
+}
+
+__global__ void __launch_bounds__(64) k_p2p_f32(const uint32_t* __restrict__ leaf_ids,
+                                               const uint32_t* __restrict__ pptr,
+                                               const uint2* __restrict__ prange,
+                                               const uint32_t* __restrict__ cfirst,
+                                               const uint32_t* __restrict__ ccount,
+                                               const double* __restrict__ ccx,
+                                               const double* __restrict__ ccy,
+                                               const double* __restrict__ ccz,
+                                               const double4* __restrict__ body, double soft2,
+                                               const double4* __restrict__ far,
+                                               const uint32_t* __restrict__ order,
+                                               double* __restrict__ accel, double* __restrict__ pot) {
+  __shared__ float4 tile[64];
+  __shared__ uint2 rng[P2P_RMAX];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t leaf = leaf_ids[xcd_block(blockIdx.x, gridDim.x)];
+  const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
+  const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
+  const double cx = ccx[leaf], cy = ccy[leaf], cz = ccz[leaf];
+  const double4 pad = make_double4(cx + 1e18, cy + 1e18, cz + 1e18, 0.0);
+  const float s2f = (float)soft2;
+  for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
+    const uint32_t nt = min(64u, nt_all - t0);
+    uint32_t ntp = 1;
+    while (ntp < nt) ntp <<= 1;
+    const uint32_t K = 64 / ntp;
+    const uint32_t ti = lane & (ntp - 1);
+    const uint32_t sub = lane / ntp;
+    const bool tvalid = ti < nt;
+    const uint32_t si = ft + t0 + (tvalid ? ti : 0);
+    const float4 me = rel32(body[si], cx, cy, cz);
+    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
+      __syncthreads();
+      const bool in = s0 + lane < nt_all;
+      double4 v = body[in ? ft + s0 + lane : ft];
+      if (!in) v = pad;
+      tile[lane] = rel32(v, cx, cy, cz);
+      __syncthreads();
+      const int self_j = (int)(t0 + ti) - (int)s0;
+      float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
+      p2p_tile32<true>(tile, sub, K, me.x, me.y, me.z, s2f, self_j, fx, fy, fz, fp);
+      ax += fx;
+      ay += fy;
+      az += fz;
+      ph += fp;
+    }
+    for (uint32_t seg = jb; seg < je; seg += P2P_RMAX) {
+      const uint32_t nr = min(P2P_RMAX, je - seg);
+      __syncthreads();
+      for (uint32_t q = lane; q < nr; q += 64) {
+        uint2 g = prange[seg + q];
+        if (g.x == ft) g.y = 0;
+        rng[q] = g;
+      }
+      __syncthreads();
+      uint32_t r = 0, off = 0;
+      TileLoad cur = gather_tile(rng, nr, r, off, lane, body, pad);
+      while (cur.fill > 0) {
+        tile[lane] = rel32(cur.v, cx, cy, cz);
+        __syncthreads();
+        cur = gather_tile(rng, nr, r, off, lane, body, pad);
+        float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
+        p2p_tile32<false>(tile, sub, K, me.x, me.y, me.z, s2f, -1, fx, fy, fz, fp);
+        ax += fx;
+        ay += fy;
+        az += fz;
+        ph += fp;
+        __syncthreads();
+      }
+    }
+    for (uint32_t o = ntp; o < 64; o <<= 1) {
+      ax += __shfl_xor(ax, (int)o, 64);
+      ay += __shfl_xor(ay, (int)o, 64);
+      az += __shfl_xor(az, (int)o, 64);
+      ph += __shfl_xor(ph, (int)o, 64);
+    }
+    if (sub == 0 && tvalid) {
+      const double4 fv = far[si];
+      const uint32_t id = order[si];
+      accel[3 * (size_t)id] = ax + fv.x;
+      accel[3 * (size_t)id + 1] = ay + fv.y;
+      accel[3 * (size_t)id + 2] = az + fv.z;
+      if (pot) pot[id] = ph + fv.w;
+    }
+  }
+}
+
 // prange[j] = (first, count) of the source leaf of directed entry j.
 __global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__ skeys,
                                                    uint32_t nnz, uint64_t smask,
@@ -165,6 +288,19 @@ __global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__
   if (j >= nnz) return;
   const uint32_t src = (uint32_t)(skeys[j] & smask);
   prange[j] = make_uint2(cfirst[src], ccount[src]);
+}
+
+void launch_p2p_f32(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                    const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c,
+                    const double4* body, uint2* prange, double soft2, const double4* far,
+                    const uint32_t* order, double* accel, double* pot, hipStream_t s) {
+  if (nleaves == 0) return;
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  if (nnz)
+    hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
+                       c.first, c.count, prange);
+  hipLaunchKernelGGL(k_p2p_f32, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, prange, c.first,
+                     c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot);
 }
 
 void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,

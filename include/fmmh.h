@@ -48,7 +48,9 @@ extern "C" {
 
 /* Precision forms */
 #define FMMH_PREC_FP64 0      /* plain FP64 everywhere (the drop-in) */
-#define FMMH_PREC_FP32BULK 1  /* reserved for the mixed-precision form */
+#define FMMH_PREC_FP32BULK 1  /* mixed precision: P2P pair terms in FP32 on positions
+                                 relative to the target leaf's centre, FP64 tile and
+                                 final accumulation; everything else FP64 */
 
 /* Flags */
 #define FMMH_FLAG_TIMING 1u   /* record per-stage device times (hipEvents) */

@@ -280,3 +280,24 @@ def test_large_theta_mac_properties(fa):
     rel = fa.rel_errors(r.accel[ids], a_ref)
     say(f"  median rel err {np.median(rel):.3e} p99 {np.percentile(rel, 99):.3e}")
     assert np.median(rel) < 1e-5
+
+
+@pytest.mark.parametrize("soft,n_leaf", [(1e-3, 32), (0.0, 96), (1e-2, 64)])
+def test_fp32bulk_form_accuracy(fa, soft, n_leaf):
+    """FP32-bulk form (P2P pair terms in FP32 on leaf-centred positions, FP64
+    tile and final sums): within 5e-6 of the FP64 path per body and no worse
+    than it against the FP64 direct sum (bars of the mixed form)."""
+    pos, mass = O.plummer(20000, seed=71)
+    say(f"[fp32bulk] soft={soft} n_leaf={n_leaf}")
+    kw = dict(p=8, eps=math.inf, n_leaf=n_leaf, theta_max=0.5, softening=soft)
+    r64 = fa.Fmm(None, None, **kw).accel(pos, mass)
+    r32 = fa.Fmm(None, None, precision="fp32bulk", **kw).accel(pos, mass)
+    an = np.linalg.norm(r64.accel, axis=1)
+    d = np.linalg.norm(r32.accel - r64.accel, axis=1) / an
+    assert d.max() < 5e-6 and np.median(d) < 5e-7, (d.max(), np.median(d))
+    assert np.all(np.abs(r32.pot - r64.pot) <= 5e-6 * np.abs(r64.pot))
+    ids = fa.sample_ids(len(mass), 2000, 7)
+    a_ref, _, _ = fa.direct_sampled(pos, mass, ids, softening=soft)
+    e64 = fa.rel_errors(r64.accel[ids], a_ref)
+    e32 = fa.rel_errors(r32.accel[ids], a_ref)
+    assert np.percentile(e32, 99) <= 1.2 * np.percentile(e64, 99) + 1e-6
