@@ -39,6 +39,10 @@ SOURCES = {
     "upward.hip": "off",
     "walk.hip": "off",
     "m2l.hip": "fast",
+    "m2l_p1_6.hip": "fast",
+    "m2l_p7_8.hip": "fast",
+    "m2l_p9.hip": "fast",
+    "m2l_p10.hip": "fast",
     "downward.hip": "fast",
     "near.hip": "fast",
     "generic_up.hip": "off",

@@ -227,8 +227,6 @@ static void l2p_t(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodie
     case 8: CALL(8); break;     \
     case 9: CALL(9); break;     \
     case 10: CALL(10); break;   \
-    case 11: CALL(11); break;   \
-    case 12: CALL(12); break;   \
     default: return false;      \
   }
 

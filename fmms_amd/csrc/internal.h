@@ -137,7 +137,7 @@ bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* pa
                        hipStream_t s);
 
 // ------------------------------------------- generic_up.hip / generic_far.hip
-constexpr int TEMPLATED_MAX_P = 12;  // above this the runtime-p kernels run
+constexpr int TEMPLATED_MAX_P = 10;  // above this the runtime-p kernels run
 void launch_p2m_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
                     cplx* moments, hipStream_t s);
 void launch_m2m_gen(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* moments,

@@ -57,18 +57,26 @@ __global__ void __launch_bounds__(ST) k_trace_up(const uint32_t* __restrict__ ce
   q[id] = acc;
 }
 
-// Per target: the source-side trace correction of every M2L source into the
-// local expansion (F_0^0, F_1^0, F_1^{+-1}) and the target's Lap Phi.
-__global__ void __launch_bounds__(ST) k_m2l_trace(uint32_t nc, const uint32_t* __restrict__ tptr,
-                                                 const uint64_t* __restrict__ skeys, uint64_t smask,
-                                                 DevCells c, const double* __restrict__ q,
-                                                 double soft2, int D, cplx* __restrict__ locals,
-                                                 double* __restrict__ lap) {
-  const uint32_t t = blockIdx.x * ST + threadIdx.x;
+// Per target (one wave, lanes striding over its sources): the source-side
+// trace correction of every M2L source into the local expansion (F_0^0,
+// F_1^0, F_1^{+-1}) and the target's Lap Phi.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* __restrict__ tptr,
+                                                  const uint64_t* __restrict__ skeys, uint64_t smask,
+                                                  DevCells c, const double* __restrict__ q,
+                                                  double soft2, int D, cplx* __restrict__ locals,
+                                                  double* __restrict__ lap) {
+  const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
   if (t >= nc) return;
   const double tx = c.cx[t], ty = c.cy[t], tz = c.cz[t];
   double f00 = 0.0, f10 = 0.0, f11r = 0.0, f11i = 0.0, L = 0.0;
-  for (uint32_t j = tptr[t]; j < tptr[t + 1]; j++) {
+  for (uint32_t j = tptr[t] + lane; j < tptr[t + 1]; j += 64) {
     const uint32_t src = (uint32_t)(skeys[j] & smask);
     const double rx = tx - c.cx[src], ry = ty - c.cy[src], rz = tz - c.cz[src];
     const double s2 = rx * rx + ry * ry + rz * rz + soft2;
@@ -83,6 +91,12 @@ __global__ void __launch_bounds__(ST) k_m2l_trace(uint32_t nc, const uint32_t* _
     f11r -= g * rx;
     f11i -= g * ry;
   }
+  f00 = wave_sum(f00);
+  f10 = wave_sum(f10);
+  f11r = wave_sum(f11r);
+  f11i = wave_sum(f11i);
+  L = wave_sum(L);
+  if (lane != 0) return;
   cplx* F = locals + (size_t)t * D;
   F[0].re += f00;
   if (D > 1) {
@@ -124,8 +138,8 @@ void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, 
                       DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
                       hipStream_t s) {
   const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_m2l_trace, dim3((nc + ST - 1) / ST), dim3(ST), 0, s, nc, tptr, skeys, smask,
-                     c, q, soft2, hdim(p), locals, lap);
+  hipLaunchKernelGGL(k_m2l_trace, dim3((nc + 3) / 4), dim3(256), 0, s, nc, tptr, skeys, smask, c,
+                     q, soft2, hdim(p), locals, lap);
 }
 
 void launch_lap_down(const uint32_t* cells, uint32_t cnt, const int32_t* parent, double* lap,

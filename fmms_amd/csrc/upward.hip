@@ -172,8 +172,6 @@ static void m2m_t(const uint32_t* cells, uint32_t cnt, DevCells c, cplx* moments
     case 8: CALL(8); break;    \
     case 9: CALL(9); break;    \
     case 10: CALL(10); break;  \
-    case 11: CALL(11); break;  \
-    case 12: CALL(12); break;  \
     default: return false;     \
   }
 
