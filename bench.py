@@ -272,44 +272,70 @@ def bench_ensemble(fa, hip, args):
     n = len(mass)
     d_pos, d_mass = hip.DeviceArray.from_host(pos), hip.DeviceArray.from_host(mass)
     d_acc, d_pot = hip.DeviceArray((n, 3)), hip.DeviceArray((n,))
-    n_leaf = args.n_leaf or 32
-    ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
-                 softening=soft, timing=True)
-    off = ctx.set_systems(sizes)
-    run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(1, args.warmup))
-    log(f"[bench] timing {args.steps} ensemble evaluations")
-    elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
-    wc = ctx.work_counts()
-    acc = d_acc.download()
+    prec = args.precision or "fp32bulk"
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    # sampled targets, 16 per system, against each system's FP64 direct sum
+    tgt, a_ref = [], []
+    if not args.no_error:
+        log("[bench] FP64 direct sums on 16 sampled targets per system")
+        for i, s in enumerate(systems):
+            ids = fa.sample_ids(s.n, 16, 7 + i)
+            a_ref.append(fa.direct_sampled(s.pos, s.mass, ids, softening=soft)[0])
+            tgt.append(off[i] + ids.astype(np.int64))
+        tgt, a_ref = np.concatenate(tgt), np.concatenate(a_ref)
+
+    def measure(theta, n_leaf, steps, timing=False):
+        ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
+                     softening=soft, timing=timing, precision=prec)
+        ctx.set_systems(sizes)
+        run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, 1)
+        el, lat, st = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, steps)
+        wc = ctx.work_counts()
+        ctx.close()
+        acc = d_acc.download()
+        rel = fa.rel_errors(acc[tgt], a_ref) if len(tgt) else None
+        return el, lat, st, wc, rel
+
+    # fastest (theta, n_leaf) meeting the error bar (p fixed by the workload)
+    best, tried = None, []
+    for theta in ([args.theta] if args.n_leaf else [0.5, 0.45, 0.4]):
+        for nl in ([args.n_leaf] if args.n_leaf else [32, 64]):
+            el, _, _, _, rel = measure(theta, nl, 2)
+            p99 = float(np.percentile(rel, 99)) if rel is not None else 0.0
+            ok = p99 <= args.max_rel_err
+            tried.append({"theta": theta, "n_leaf": nl, "ms": 1e3 * el / 2, "rel_p99": p99,
+                          "valid": ok})
+            log(f"[autotune] theta={theta} n_leaf={nl}: {1e3 * el / 2:.1f} ms, rel p99 {p99:.2e}")
+            if ok and (best is None or el < best[2]):
+                best = (theta, nl, el)
+    if best is None:
+        t = min(tried, key=lambda r: r["rel_p99"])
+        best = (t["theta"], t["n_leaf"], None)
+    theta, n_leaf = best[0], best[1]
+    log(f"[bench] timing {args.steps} ensemble evaluations (theta={theta}, n_leaf={n_leaf}, {prec})")
+    elapsed, lat, stages, wc, rel = measure(theta, n_leaf, args.steps, timing=True)
     avg = {kk: float(np.mean(v)) for kk, v in stages.items()}
     roof_stages = stage_rooflines(avg, wc, args.p)
     dom = max(roof_stages, key=lambda kk: roof_stages[kk]["ms"])
     roof = {"kernel": dom, "bound": "valu", "achieved": round(roof_stages[dom]["achieved"], 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(roof_stages[dom]["frac"], 4),
             "traffic": None, "avg_launch_ms": round(roof_stages[dom]["ms"], 4)}
-    # one system at a time (the reference's model: one Fmm per system)
+    # one system at a time (the reference's model: one Fmm per system), same parameters
     log("[bench] one-system-at-a-time pass")
     seq = 0.0
-    rel = []
     for i, s in enumerate(systems):
-        a, b = int(off[i]), int(off[i + 1])
-        one = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
-                     softening=soft)
+        one = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
+                     softening=soft, precision=prec)
         dp, dm = hip.DeviceArray.from_host(s.pos), hip.DeviceArray.from_host(s.mass)
         da, dq = hip.DeviceArray((s.n, 3)), hip.DeviceArray((s.n,))
         one.accel_device(s.n, dp.ptr, dm.ptr, da.ptr, dq.ptr)
         el, _, _ = run_steps(fa, hip, one, s.n, dp, dm, da, dq, 2)
         seq += el / 2
         one.close()
-        if not args.no_error:  # 16 sampled targets per system against its direct sum
-            ids = fa.sample_ids(s.n, 16, 7 + i)
-            a_ref, _, _ = fa.direct_sampled(s.pos, s.mass, ids, softening=soft)
-            rel.append(fa.rel_errors(acc[a + ids.astype(np.int64)], a_ref))
         for x in (dp, dm, da, dq):
             x.free()
     err = None
-    if rel:
-        rel = np.concatenate(rel)
+    if rel is not None:
         err = {"targets": int(rel.size), "rel_median": float(np.median(rel)),
                "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max())}
         log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
@@ -320,10 +346,13 @@ def bench_ensemble(fa, hip, args):
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Plummer spheres and exponential disks)",
-        "config": {"workload": f"ensemble{k}_N{n}_p{args.p}_theta{args.theta}_soft{soft}",
+        "config": {"workload": f"ensemble{k}_N{n}_p{args.p}_theta{theta}_soft{soft}_{prec}",
                    "systems": k, "n": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
-                   "p": args.p, "theta": args.theta, "softening": soft, "n_leaf": n_leaf,
-                   "precision": "fp64"},
+                   "p": args.p, "theta": theta, "softening": soft, "n_leaf": n_leaf,
+                   "precision": prec,
+                   "selection": "fastest (theta, n_leaf) over {0.5, 0.45, 0.4} x {32, 64} "
+                                f"with rel err p99 <= {args.max_rel_err:g}"},
+        "autotune": tried,
         "one_at_a_time": {"value": n / seq, "ms_per_pass": 1e3 * seq,
                           "speedup_batched": (n * args.steps / elapsed) / (n / seq)},
         "error": err, "roofline": roof, "stages_ms": {kk: round(v, 4) for kk, v in avg.items()},
@@ -352,14 +381,15 @@ def bench_serve(fa, hip, args):
     log(f"[bench] {len(reqs)} requests, {sum(r.n for r in reqs)} bodies, "
         f"{total_steps} body-steps, arrivals over {t:.1f} s")
     n_leaf = args.n_leaf or 32
+    prec = args.precision or "fp32bulk"
     srv = serve.Server(p=args.p, theta=args.theta, softening=1e-3, n_leaf=n_leaf, dt=1e-4,
-                       hbm_budget=200e9)
+                       hbm_budget=200e9, precision=prec)
     wall = srv.run(reqs)
     st = srv.stats()
     lat = np.array([r.finished - r.arrival for r in reqs])
     log(f"[bench] served in {wall:.2f} s: step p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms")
-    # one system at a time, first come first served (the same requests)
-    log("[bench] one-system-at-a-time pass")
+    # plain FP64, one system at a time, first come first served (the same requests)
+    log("[bench] one-system-at-a-time pass (plain FP64)")
     alone = 0.0
     alone_ms = []
     for r in reqs:
@@ -377,7 +407,7 @@ def bench_serve(fa, hip, args):
         "config": {"workload": f"serve_{len(reqs)}req_rate{args.rate}_p{args.p}_theta{args.theta}",
                    "requests": len(reqs), "rate_per_s": args.rate, "p": args.p,
                    "theta": args.theta, "softening": 1e-3, "n_leaf": n_leaf, "dt": 1e-4,
-                   "hbm_budget_bytes": 200e9, "precision": "fp64"},
+                   "hbm_budget_bytes": 200e9, "precision": prec},
         "serving": {"wall_s": wall, "body_steps": total_steps,
                     "wall_body_forces_per_s": total_steps / wall,
                     "request_latency_p50_s": float(np.percentile(lat, 50)),
@@ -413,6 +443,8 @@ def main():
     ap.add_argument("--workload", choices=["plummer", "halos", "ensemble", "serve"],
                     default="plummer")
     ap.add_argument("--systems", type=int, default=256, help="ensemble: number of systems")
+    ap.add_argument("--precision", default=None,
+                    help="ensemble / serve: fp64 or fp32bulk (default: fp32bulk)")
     ap.add_argument("--rate", type=float, default=20.0,
                     help="serve: Poisson arrival rate (requests per second)")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")

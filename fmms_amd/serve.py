@@ -78,8 +78,9 @@ class Server:
     FIELDS = (("pos", 3), ("vel", 3), ("mass", 1), ("acc", 3), ("pot", 1))
 
     def __init__(self, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4,
-                 hbm_budget=200e9):
-        self.kw = dict(p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening)
+                 hbm_budget=200e9, precision="fp64"):
+        self.kw = dict(p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening,
+                       precision=precision)
         self.dt = dt
         self.hbm_budget = float(hbm_budget)
         self.ctx = Fmm(None, None, **self.kw)
