@@ -154,7 +154,7 @@ def bench_halos(fa, hip, args):
     d_acc = hip.DeviceArray((n, 3))
     d_pot = hip.DeviceArray((n,))
     n_leaf = args.n_leaf or 64
-    mk = lambda timing=False: fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf,
+    mk = lambda timing=False: fa.Fmm(None, None, p=args.p, eps=args.eps_mac, n_leaf=n_leaf,
                                       theta_max=args.theta, softening=soft, timing=timing)
 
     def trajectory(ctx, steps):
@@ -234,6 +234,7 @@ def bench_halos(fa, hip, args):
         "data": "synthetic (seeded Hernquist halos in a uniform cube)",
         "config": {"workload": f"halos64_cube_N{n}_p{args.p}_theta{args.theta}_soft{soft}_kdk",
                    "n": n, "p": args.p, "theta": args.theta, "softening": soft,
+                   "eps_mac": "inf" if math.isinf(args.eps_mac) else args.eps_mac,
                    "n_leaf": n_leaf, "dt": args.dt, "refresh_every": args.refresh,
                    "drift_tol": 0.25, "graph": True, "precision": "fp64"},
         "leapfrog": lf,
@@ -451,6 +452,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=10,
                     help="halos: rebuild the tree and lists every this many steps")
     ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--eps-mac", type=float, default=math.inf,
+                    help="halos: error-controlled MAC tolerance beside theta (inf = theta only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device (a copy kernel of known "
                          "size, for scaling FETCH_SIZE/WRITE_SIZE counters)")

@@ -322,13 +322,6 @@ struct fmmh_ctx {
     ensure_scan(nc);
     HCHECK(hipMemsetAsync(c_children.p, 0xFF, (size_t)nc * 32, s));
     launch_scatter_cells(bfs(), nc, (uint32_t)prm.n_leaf, vlev, cells(), (uint32_t*)lvl_cells.p, s);
-    for (int L = (int)max_level; L >= 0; L--)
-      launch_geometry((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(),
-                      bodies(), prm.center_mode, s);
-    for (int L = 0; L <= (int)max_level; L++)
-      launch_ropes((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(), s);
-    ensure(walkrec, (size_t)nc * walk_rec_bytes());
-    launch_pack_walk(nc, cells(), walkrec.p, s);
     launch_leaf_flags(cells(), nc, (uint32_t*)leaf_flags.p, s);
     uint32_t* tot = nullptr;
     scan_exclusive_u32((uint32_t*)leaf_flags.p, (uint32_t*)leaf_off.p, nc, (uint32_t*)scan_tmp.p, s,
@@ -336,6 +329,16 @@ struct fmmh_ctx {
     launch_leaf_compact(cells(), nc, (uint32_t*)leaf_off.p, (uint32_t*)leaf_ids.p,
                         (uint32_t*)leaf_of_slot.p, s);
     nleaves = read1(tot);
+    // geometry: every leaf at once (from its bodies), then internal cells
+    // bottom-up level by level (from their children)
+    launch_geometry((uint32_t*)leaf_ids.p, nleaves, cells(), bodies(), prm.center_mode, s, 1);
+    for (int L = (int)max_level - 1; L >= 0; L--)
+      launch_geometry((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(),
+                      bodies(), prm.center_mode, s, 2);
+    for (int L = 0; L <= (int)max_level; L++)
+      launch_ropes((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(), s);
+    ensure(walkrec, (size_t)nc * walk_rec_bytes());
+    launch_pack_walk(nc, cells(), walkrec.p, s);
     sbits = 1;
     while ((1ull << sbits) <= nc) sbits++;
     built = true;

@@ -76,8 +76,9 @@ void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8
                     hipStream_t s);
 void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, int vlev, DevCells c,
                           uint32_t* lvl_cells, hipStream_t s);
+// which = 0: all cells of the list, 1: its leaves only, 2: its internal cells only.
 void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
-                     int center_mode, hipStream_t s);
+                     int center_mode, hipStream_t s, int which);
 void launch_ropes(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, hipStream_t s);
 void launch_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags, hipStream_t s);
 void launch_leaf_compact(DevCells c, uint32_t nc, const uint32_t* off, uint32_t* leaf_ids,

@@ -19,7 +19,10 @@
 
 namespace fmmh {
 
-constexpr int M2L_CH = 16;
+#ifndef FMMH_M2L_CH
+#define FMMH_M2L_CH 16
+#endif
+constexpr int M2L_CH = FMMH_M2L_CH;  // sources per chunk (one thread's serial work)
 
 __device__ __forceinline__ void cmac(cplx& f, double ar, double ai, double br, double bi) {
   f.re = fma(ar, br, f.re);

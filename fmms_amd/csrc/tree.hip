@@ -499,11 +499,13 @@ __device__ void ritter(int nitems, Get get, double out[3]) {
 }
 
 // tree.rs:152-227, one thread per cell of a level (children done first).
+// which = 0: every cell of the list; 1: only its leaves; 2: only its internal cells.
 __global__ void __launch_bounds__(TB) k_geometry(const uint32_t* __restrict__ cells, uint32_t cnt,
-                                                DevCells c, DevBodies b, int mode) {
+                                                DevCells c, DevBodies b, int mode, int which) {
   uint32_t i = blockIdx.x * TB + threadIdx.x;
   if (i >= cnt) return;
   uint32_t id = cells[i];
+  if ((which == 1 && !c.leaf[id]) || (which == 2 && c.leaf[id])) return;
   double ms = 0.0;
   double cen[3];
   double rho = 0.0;
@@ -580,10 +582,10 @@ __global__ void __launch_bounds__(TB) k_geometry(const uint32_t* __restrict__ ce
 }
 
 void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
-                     int center_mode, hipStream_t s) {
+                     int center_mode, hipStream_t s, int which) {
   if (cnt == 0) return;
   hipLaunchKernelGGL(k_geometry, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, lvl_cells, cnt, c, b,
-                     center_mode);
+                     center_mode, which);
 }
 
 // Ropes of the mirrored DFS (children visited 7..0): the cell visited after
