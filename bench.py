@@ -140,7 +140,7 @@ def stage_rooflines(stages_ms, wc, p):
 
 def bench_halos(fa, hip, args):
     n = args.n if args.n != (1 << 20) else 1 << 24
-    soft = 1e-4
+    soft = args.halo_softening
     log(f"[bench] generating {n} bodies: 64 Hernquist halos in a unit cube")
     t0 = time.perf_counter()
     s = fa.ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
@@ -452,6 +452,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=10,
                     help="halos: rebuild the tree and lists every this many steps")
     ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--halo-softening", type=float, default=1e-4,
+                    help="halos: Plummer softening")
     ap.add_argument("--eps-mac", type=float, default=math.inf,
                     help="halos: error-controlled MAC tolerance beside theta (inf = theta only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -362,13 +362,7 @@ __global__ void __launch_bounds__(64) k_l2p_gen(int p, const uint32_t* __restric
       }
     }
     double4 o = make_double4(-p11r, -p11i, -p10r, p00);
-    if (lap) {  // target-side trace term of the softened far field (x = centre - body)
-      const double L = lap[leaf];
-      o.x -= x * L * (1.0 / 3.0);
-      o.y -= y * L * (1.0 / 3.0);
-      o.z -= z * L * (1.0 / 3.0);
-      o.w += r2 * L * (1.0 / 6.0);
-    }
+    if (lap) l2p_trace4(lap + 4 * (size_t)leaf, -x, -y, -z, o);  // softened far field
     far[k] = o;
   }
 }

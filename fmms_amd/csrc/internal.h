@@ -172,8 +172,18 @@ void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, con
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
                       DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
                       hipStream_t s);
-void launch_lap_down(const uint32_t* cells, uint32_t cnt, const int32_t* parent, double* lap,
-                     hipStream_t s);
+void launch_lap_down(const uint32_t* cells, uint32_t cnt, DevCells c, double* lap, hipStream_t s);
+// Target-side trace terms at L2P (lap4 = L, G at the leaf centre; d = body - centre).
+__device__ __forceinline__ void l2p_trace4(const double* lap4, double dx, double dy, double dz,
+                                           double4& o) {
+  const double L = lap4[0], g0 = lap4[1], g1 = lap4[2], g2 = lap4[3];
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  const double dg = dx * g0 + dy * g1 + dz * g2;
+  o.w += d2 * L * (1.0 / 6.0) + d2 * dg * 0.1;
+  o.x += dx * L * (1.0 / 3.0) + (2.0 * dx * dg + d2 * g0) * 0.1;
+  o.y += dy * L * (1.0 / 3.0) + (2.0 * dy * dg + d2 * g1) * 0.1;
+  o.z += dz * L * (1.0 / 3.0) + (2.0 * dz * dg + d2 * g2) * 0.1;
+}
 
 // -------------------------------------------------------------- near.hip
 void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,

@@ -7,8 +7,11 @@
 // ones are kept (Lap K = -3 eps^2 s^{-5}):
 //   * source side, (q/6) Lap K, q = sum m |x - z|^2 of the source cell, added
 //     to each M2L target's local expansion to first order (F_0^0 and F_1^m);
-//   * target side, (|dx|^2 / 6) Lap Phi, Lap Phi = sum over the cell's (and
-//     its ancestors') M2L sources of M Lap K at the centre, applied at L2P.
+//   * target side, the second- and third-order trace parts of the Taylor
+//     series about the target centre, (|d|^2 / 6) L + (|d|^2 / 10) d.G, with
+//     L = Lap Phi and G = grad Lap Phi of the cell's (and its ancestors')
+//     M2L sources' monopole fields at the centre (4 doubles per cell),
+//     applied at L2P.
 // Oracle and device apply the same rule; eps = 0 skips all of it.
 #include "common.h"
 #include "internal.h"
@@ -75,7 +78,7 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
   const uint32_t lane = threadIdx.x & 63;
   if (t >= nc) return;
   const double tx = c.cx[t], ty = c.cy[t], tz = c.cz[t];
-  double f00 = 0.0, f10 = 0.0, f11r = 0.0, f11i = 0.0, L = 0.0;
+  double f00 = 0.0, f10 = 0.0, f11r = 0.0, f11i = 0.0, L = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
   for (uint32_t j = tptr[t] + lane; j < tptr[t + 1]; j += 64) {
     const uint32_t src = (uint32_t)(skeys[j] & smask);
     const double rx = tx - c.cx[src], ry = ty - c.cy[src], rz = tz - c.cz[src];
@@ -83,7 +86,12 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     const double is = 1.0 / sqrt(s2);
     const double is2 = is * is;
     const double is5 = is2 * is2 * is;
-    L -= 3.0 * soft2 * c.mass[src] * is5;
+    const double ms = c.mass[src];
+    L -= 3.0 * soft2 * ms * is5;             // Lap of the source monopole's field
+    const double gl = 15.0 * soft2 * ms * is5 / s2;  // its gradient / r
+    gx += gl * rx;
+    gy += gl * ry;
+    gz += gl * rz;
     const double cc = 0.5 * q[src] * soft2 * is5;  // -(q/6) Lap K at the target centre
     const double g = 5.0 * cc / s2;                 // its gradient / r
     f00 -= cc;
@@ -96,6 +104,9 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
   f11r = wave_sum(f11r);
   f11i = wave_sum(f11i);
   L = wave_sum(L);
+  gx = wave_sum(gx);
+  gy = wave_sum(gy);
+  gz = wave_sum(gz);
   if (lane != 0) return;
   cplx* F = locals + (size_t)t * D;
   F[0].re += f00;
@@ -106,18 +117,28 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     F[1].re -= f11r;          // (1, -1) = -conj(F_1^1)
     F[1].im += f11i;
   }
-  lap[t] = L;
+  lap[4 * (size_t)t] = L;
+  lap[4 * (size_t)t + 1] = gx;
+  lap[4 * (size_t)t + 2] = gy;
+  lap[4 * (size_t)t + 3] = gz;
 }
 
-// lap of each cell of a level += lap of its parent (levels top-down).
+// Each cell of a level adds its parent's linear Laplacian field shifted to
+// its centre: L += L_p + G_p . (z - z_p), G += G_p (levels top-down).
 __global__ void __launch_bounds__(ST) k_lap_down(const uint32_t* __restrict__ cells, uint32_t cnt,
-                                                const int32_t* __restrict__ parent,
-                                                double* __restrict__ lap) {
+                                                DevCells c, double* __restrict__ lap) {
   const uint32_t i = blockIdx.x * ST + threadIdx.x;
   if (i >= cnt) return;
   const uint32_t id = cells[i];
-  const int32_t pa = parent[id];
-  if (pa >= 0) lap[id] += lap[pa];
+  const int32_t pa = c.parent[id];
+  if (pa < 0) return;
+  const double* lp = lap + 4 * (size_t)pa;
+  double* lc = lap + 4 * (size_t)id;
+  lc[0] += lp[0] + lp[1] * (c.cx[id] - c.cx[pa]) + lp[2] * (c.cy[id] - c.cy[pa]) +
+           lp[3] * (c.cz[id] - c.cz[pa]);
+  lc[1] += lp[1];
+  lc[2] += lp[2];
+  lc[3] += lp[3];
 }
 
 void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
@@ -142,10 +163,9 @@ void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, 
                      q, soft2, hdim(p), locals, lap);
 }
 
-void launch_lap_down(const uint32_t* cells, uint32_t cnt, const int32_t* parent, double* lap,
-                     hipStream_t s) {
+void launch_lap_down(const uint32_t* cells, uint32_t cnt, DevCells c, double* lap, hipStream_t s) {
   if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lap_down, dim3((cnt + ST - 1) / ST), dim3(ST), 0, s, cells, cnt, parent, lap);
+  hipLaunchKernelGGL(k_lap_down, dim3((cnt + ST - 1) / ST), dim3(ST), 0, s, cells, cnt, c, lap);
 }
 
 }  // namespace fmmh

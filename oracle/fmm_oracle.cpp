@@ -1087,7 +1087,8 @@ void orc_binomials(int p, double* out) {
 
 static void trace_pair(int p, const double* zs, const double* zt, double q, double mass,
                        double soft2, C* f, double* lap);
-static void l2p_trace(const double* cen, const double* x, double L, double* pot, double* a);
+static void l2p_trace(const double* cen, const double* x, const double* L4, double* pot,
+                      double* a);
 
 void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], const double* ma,
                          const double* mb, double eps2, double* la, double* lb) {
@@ -1096,13 +1097,13 @@ void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], co
 }
 
 void orc_soft_trace_pair(int p, const double cen_src[3], const double cen_dst[3], double q,
-                         double mass, double eps2, double* f_dst, double* lap_dst) {
-  trace_pair(p, cen_src, cen_dst, q, mass, eps2, (C*)f_dst, lap_dst);
+                         double mass, double eps2, double* f_dst, double* lap4_dst) {
+  trace_pair(p, cen_src, cen_dst, q, mass, eps2, (C*)f_dst, lap4_dst);
 }
 
-void orc_l2p_trace(const double cen[3], const double pos[3], double lap, double* pot,
+void orc_l2p_trace(const double cen[3], const double pos[3], const double* lap4, double* pot,
                    double acc[3]) {
-  l2p_trace(cen, pos, lap, pot, acc);
+  l2p_trace(cen, pos, lap4, pot, acc);
 }
 
 void orc_m2l_mutual(int p, const double cen_a[3], const double cen_b[3], const double* ma,
@@ -1274,9 +1275,10 @@ void orc_get_bbox(const orc_ctx* t, double lo[3], double hi[3], double* inv_side
 //    Lap K = -3 eps^2 s^{-5}: added to every M2L target's local expansion to
 //    first order (value into F_0^0, gradient into F_1^m; l2p_into gives
 //    pot = F00 - F10 dz - Re F11 dx - Im F11 dy);
-//  * target side, (|dx|^2 / 6) Lap Phi: lap[cell] = sum over its M2L
-//    sources of M Lap K at the centre, inherited by the children (L2L) and
-//    applied at L2P as pot += |dx|^2 lap / 6, a += dx lap / 3.
+//  * target side, (|d|^2 / 6) Lap Phi + (|d|^2 / 10) d.grad(Lap Phi): per
+//    cell the Laplacian of its M2L sources' monopole fields and its gradient
+//    at the centre (lap[4c..4c+3]), Taylor-shifted to the children (L2L) and
+//    applied at L2P.
 // q is summed directly from the bodies of each cell.
 // One directed M2L pair's trace terms: source-side correction into the
 // target's local expansion f, Lap of the source's monopole field into *lap.
@@ -1286,7 +1288,11 @@ static void trace_pair(int p, const double* zs, const double* zt, double q, doub
   double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + soft2;
   double is = 1.0 / std::sqrt(s2);
   double is5 = is * is * is * is * is;
-  *lap += -3.0 * soft2 * mass * is5;
+  lap[0] += -3.0 * soft2 * mass * is5;  // Lap of the source's monopole field
+  double gl = 15.0 * soft2 * mass * is5 / s2;  // and its gradient / r
+  lap[1] += gl * r[0];
+  lap[2] += gl * r[1];
+  lap[3] += gl * r[2];
   double c = 0.5 * q * soft2 * is5;  // -(q/6) Lap K at zt
   double g[3] = {5.0 * c * r[0] / s2, 5.0 * c * r[1] / s2, 5.0 * c * r[2] / s2};  // its gradient
   f[idx(0, 0)].re -= c;
@@ -1299,13 +1305,16 @@ static void trace_pair(int p, const double* zs, const double* zt, double q, doub
   }
 }
 
-// Target-side trace term at L2P: pot += |dx|^2 L / 6, a += dx L / 3.
-static void l2p_trace(const double* cen, const double* x, double L, double* pot, double* a) {
-  double dx = x[0] - cen[0], dy = x[1] - cen[1], dz = x[2] - cen[2];
-  *pot += (dx * dx + dy * dy + dz * dz) * L / 6.0;
-  a[0] += dx * L / 3.0;
-  a[1] += dy * L / 3.0;
-  a[2] += dz * L / 3.0;
+// Target-side trace terms at L2P, from the far field's Laplacian L and its
+// gradient G at the cell centre: the second- and third-order trace parts of
+// the Taylor series, pot += |d|^2 L / 6 + |d|^2 (d.G) / 10, a = their gradient.
+static void l2p_trace(const double* cen, const double* x, const double* L4, double* pot,
+                      double* a) {
+  double d[3] = {x[0] - cen[0], x[1] - cen[1], x[2] - cen[2]};
+  double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double L = L4[0], dg = d[0] * L4[1] + d[1] * L4[2] + d[2] * L4[3];
+  *pot += d2 * L / 6.0 + d2 * dg / 10.0;
+  for (int k = 0; k < 3; k++) a[k] += d[k] * L / 3.0 + (2.0 * d[k] * dg + d2 * L4[1 + k]) / 10.0;
 }
 
 static void softened_trace(orc_ctx& t, int p, double soft2) {
@@ -1321,10 +1330,10 @@ static void softened_trace(orc_ctx& t, int p, double soft2) {
     }
     q[c] = acc;
   }
-  t.lap.assign(nc, 0.0);
+  t.lap.assign(4 * nc, 0.0);  // per cell: Lap Phi and its gradient at the centre
   auto add = [&](size_t src, size_t dst) {
     trace_pair(p, t.cells[src].cen, t.cells[dst].cen, q[src], t.cells[src].mass, soft2,
-               t.locals.data() + dst * d, &t.lap[dst]);
+               t.locals.data() + dst * d, &t.lap[4 * dst]);
   };
   for (size_t k = 0; k + 1 < t.m2l.size(); k += 2) {
     add(t.m2l[k], t.m2l[k + 1]);
@@ -1543,7 +1552,16 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
           if (ch < 0) continue;
           l2l_into(p, cell.cen, t->cells[ch].cen, t->locals.data() + c * d,
                    t->locals.data() + (size_t)ch * d, u.data());
-          if (!t->lap.empty()) t->lap[ch] += t->lap[c];
+          if (!t->lap.empty()) {  // shift the parent's linear Laplacian field to the child
+            const double* lp = &t->lap[4 * c];
+            double* lc = &t->lap[4 * (size_t)ch];
+            const double* zc = t->cells[ch].cen;
+            lc[0] += lp[0] + lp[1] * (zc[0] - cell.cen[0]) + lp[2] * (zc[1] - cell.cen[1]) +
+                     lp[3] * (zc[2] - cell.cen[2]);
+            lc[1] += lp[1];
+            lc[2] += lp[2];
+            lc[3] += lp[3];
+          }
         }
       }
     });
@@ -1563,7 +1581,7 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
         double a[3] = {0.0, 0.0, 0.0};
         double x[3] = {t->bodies.x[k], t->bodies.y[k], t->bodies.z[k]};
         l2p_into(p, cell.cen, x, t->locals.data() + c * d, &p0, a, u.data());
-        if (!t->lap.empty()) l2p_trace(cell.cen, x, t->lap[c], &p0, a);
+        if (!t->lap.empty()) l2p_trace(cell.cen, x, &t->lap[4 * c], &p0, a);
         pot[id] += t->bodies.pot[k] + p0;
         accel[3 * id + 0] += t->bodies.ax[k] + a[0];
         accel[3 * id + 1] += t->bodies.ay[k] + a[1];

@@ -49,7 +49,7 @@ def _load():
         "orc_m2l_rotation": (None, [ct.c_int, _pd, _pd, _pd, _pd, _pd, _pd]),
         "orc_m2l_mutual_soft": (None, [ct.c_int, _pd, _pd, _pd, _pd, _d, _pd, _pd]),
         "orc_soft_trace_pair": (None, [ct.c_int, _pd, _pd, _d, _d, _d, _pd, _pd]),
-        "orc_l2p_trace": (None, [_pd, _pd, _d, _pd, _pd]),
+        "orc_l2p_trace": (None, [_pd, _pd, _pd, _pd, _pd]),
         "orc_rot_blocksum": (_sz, [ct.c_int]),
         "orc_rot_swap": (None, [ct.c_int, _pd]),
         "orc_l2l": (None, [ct.c_int, _pd, _pd, _pd, _pd]),
@@ -273,18 +273,18 @@ def m2l_mutual_soft(p, cen_a, cen_b, ma, mb, eps2):
 
 def soft_trace_pair(p, cen_src, cen_dst, q, mass, eps2, f_dst):
     """Adds the source-side trace correction into f_dst (in place); returns
-    the Lap contribution of the source monopole at the target centre."""
-    lap = np.zeros(1)
+    [Lap, grad Lap] of the source monopole's field at the target centre."""
+    lap = np.zeros(4)
     a, b = _f64(cen_src), _f64(cen_dst)
     lib().orc_soft_trace_pair(p, _p(a), _p(b), float(q), float(mass), float(eps2),
                               f_dst.ctypes.data_as(_pd), _p(lap))
-    return lap[0]
+    return lap
 
 
-def l2p_trace(cen, pos, lap):
+def l2p_trace(cen, pos, lap4):
     pot = np.zeros(1)
     acc = np.zeros(3)
-    lib().orc_l2p_trace(_p(_f64(cen)), _p(_f64(pos)), float(lap), _p(pot), _p(acc))
+    lib().orc_l2p_trace(_p(_f64(cen)), _p(_f64(pos)), _p(_f64(lap4)), _p(pot), _p(acc))
     return pot[0], acc
 
 

@@ -512,7 +512,9 @@ def test_softened_far_field_against_exact_sum(eps):
     _, F = O.m2l_mutual_soft(p, zA, zB, M, zero, eps * eps)
     Fc = F.copy()
     lap = O.soft_trace_pair(p, zA, zB, q, mA.sum(), eps * eps, Fc)
-    assert lap == pytest.approx(-3 * eps * eps * mA.sum() * (1 + eps * eps) ** -2.5, rel=1e-14)
+    s2 = 1 + eps * eps
+    assert lap[0] == pytest.approx(-3 * eps * eps * mA.sum() * s2 ** -2.5, rel=1e-14)
+    assert np.allclose(lap[1:], 15 * eps * eps * mA.sum() * s2 ** -3.5 * zB, rtol=1e-14)
     err_with, err_without = [], []
     for db in dB:
         x = zB + db
