@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
                                            const double* __restrict__ bz,
                                            const cplx* __restrict__ locals,
                                            const double* __restrict__ lap,
+                                           const int32_t* __restrict__ parent,
                                            double4* __restrict__ far) {
   constexpr int D = hdim(P), NF = hnpos(P);
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
     }
   }
   double4 o = make_double4(-p11r, -p11i, -p10r, p00);
-  if (lap) l2p_trace4(lap + 4 * (size_t)c, -x, -y, -z, o);  // softened far field (x = centre - body)
+  if (lap) l2p_trace_walk(lap, parent, cx, cy, cz, c, bx[k], by[k], bz[k], o);  // softened far field
   far[k] = o;
 }
 
@@ -206,7 +207,7 @@ template <int P>
 static void l2p_t(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
                   const cplx* locals, const double* lap, double4* far, hipStream_t s) {
   hipLaunchKernelGGL(k_l2p<P>, dim3((n + 63) / 64), dim3(64), 0, s, n, leaf_of_slot, c.cx, c.cy,
-                     c.cz, b.x, b.y, b.z, locals, lap, far);
+                     c.cz, b.x, b.y, b.z, locals, lap, c.parent, far);
 }
 
 #define FMMH_P_SWITCH2(p, CALL) \

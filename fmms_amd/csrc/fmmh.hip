@@ -103,7 +103,7 @@ struct fmmh_ctx {
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
   Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
-  Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; Lap Phi and its gradient per cell
+  Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; L, G, H per cell
   int rot_p = -1;
   std::vector<char> rot_struct;
   Buf direct_ids, direct_out;
@@ -493,7 +493,6 @@ struct fmmh_ctx {
         launch_l2l(p, lc, cnt, c, loc, s);
       else
         launch_l2l_gen(p, lc, cnt, c, loc, s);
-      if (lap) launch_lap_down(lc, cnt, c, lap, s);
     }
     mark(ST_L2P);
     if (!gen)
@@ -519,7 +518,7 @@ struct fmmh_ctx {
     ensure(locals, nc * D * sizeof(cplx));
     if (prm.softening > 0.0) {
       ensure(ctrace, nc * 8);
-      ensure(clap, nc * 32);
+      ensure(clap, nc * SOFT_LAP_WORDS * sizeof(double));
     }
     upward(pos, mass, regather);
     mark(ST_PREPASS);

@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(64) k_l2p_gen(int p, const uint32_t* __restric
       }
     }
     double4 o = make_double4(-p11r, -p11i, -p10r, p00);
-    if (lap) l2p_trace4(lap + 4 * (size_t)leaf, -x, -y, -z, o);  // softened far field
+    if (lap) l2p_trace_walk(lap, c.parent, c.cx, c.cy, c.cz, leaf, b.x[k], b.y[k], b.z[k], o);
     far[k] = o;
   }
 }

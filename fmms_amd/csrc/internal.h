@@ -172,17 +172,36 @@ void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, con
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
                       DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
                       hipStream_t s);
-void launch_lap_down(const uint32_t* cells, uint32_t cnt, DevCells c, double* lap, hipStream_t s);
-// Target-side trace terms at L2P (lap4 = L, G at the leaf centre; d = body - centre).
-__device__ __forceinline__ void l2p_trace4(const double* lap4, double dx, double dy, double dz,
-                                           double4& o) {
-  const double L = lap4[0], g0 = lap4[1], g1 = lap4[2], g2 = lap4[3];
-  const double d2 = dx * dx + dy * dy + dz * dz;
-  const double dg = dx * g0 + dy * g1 + dz * g2;
-  o.w += d2 * L * (1.0 / 6.0) + d2 * dg * 0.1;
-  o.x += dx * L * (1.0 / 3.0) + (2.0 * dx * dg + d2 * g0) * 0.1;
-  o.y += dy * L * (1.0 / 3.0) + (2.0 * dy * dg + d2 * g1) * 0.1;
-  o.z += dz * L * (1.0 / 3.0) + (2.0 * dz * dg + d2 * g2) * 0.1;
+// Target-side trace polynomial of the softened far field, summed over a
+// leaf and all its ancestors (each cell's L, G, H about its own centre; body
+// at (bx, by, bz)): pot += |d|^2 L/6 + |d|^2 (d.G)/10 + |d|^2 (d.H.d)/28
+// - |d|^4 tr(H)/280 and acc += its gradient, d = body - centre.
+constexpr int SOFT_LAP_WORDS = 10;  // L, G (3), H (xx, yy, zz, xy, xz, yz)
+__device__ __forceinline__ void l2p_trace_walk(const double* __restrict__ lap,
+                                               const int32_t* __restrict__ parent,
+                                               const double* __restrict__ cx,
+                                               const double* __restrict__ cy,
+                                               const double* __restrict__ cz, uint32_t leaf,
+                                               double bx, double by, double bz, double4& o) {
+  for (int32_t A = (int32_t)leaf; A >= 0; A = parent[A]) {
+    const double* t = lap + (size_t)SOFT_LAP_WORDS * A;
+    const double dx = bx - cx[A], dy = by - cy[A], dz = bz - cz[A];
+    const double L = t[0];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double dg = dx * t[1] + dy * t[2] + dz * t[3];
+    const double hx = t[4] * dx + t[7] * dy + t[8] * dz;
+    const double hy = t[7] * dx + t[5] * dy + t[9] * dz;
+    const double hz = t[8] * dx + t[9] * dy + t[6] * dz;
+    const double dhd = dx * hx + dy * hy + dz * hz;
+    const double tr = t[4] + t[5] + t[6];
+    o.w += d2 * L / 6.0 + d2 * dg / 10.0 + d2 * dhd / 28.0 - d2 * d2 * tr / 280.0;
+    o.x += dx * L / 3.0 + (2.0 * dx * dg + d2 * t[1]) / 10.0 + (dx * dhd + d2 * hx) / 14.0 -
+           d2 * dx * tr / 70.0;
+    o.y += dy * L / 3.0 + (2.0 * dy * dg + d2 * t[2]) / 10.0 + (dy * dhd + d2 * hy) / 14.0 -
+           d2 * dy * tr / 70.0;
+    o.z += dz * L / 3.0 + (2.0 * dz * dg + d2 * t[3]) / 10.0 + (dz * dhd + d2 * hz) / 14.0 -
+           d2 * dz * tr / 70.0;
+  }
 }
 
 // -------------------------------------------------------------- near.hip

@@ -3,15 +3,19 @@
 // The softened kernel K = (r^2 + eps^2)^{-1/2} is not harmonic, so its Taylor
 // expansions about the source and the target centre carry, beside the
 // solid-harmonic parts that M2L / L2P evaluate (Theta scaled by (r/s)^{2n+1},
-// s^2 = r^2 + eps^2), trace terms |d|^{2k} Y_{n-2k}(d) Lap^k K. The leading
-// ones are kept (Lap K = -3 eps^2 s^{-5}):
-//   * source side, (q/6) Lap K, q = sum m |x - z|^2 of the source cell, added
-//     to each M2L target's local expansion to first order (F_0^0 and F_1^m);
-//   * target side, the second- and third-order trace parts of the Taylor
-//     series about the target centre, (|d|^2 / 6) L + (|d|^2 / 10) d.G, with
-//     L = Lap Phi and G = grad Lap Phi of the cell's (and its ancestors')
-//     M2L sources' monopole fields at the centre (4 doubles per cell),
-//     applied at L2P.
+// s^2 = r^2 + eps^2), trace terms |d|^{2k} Y_{n-2k}(d) Lap^k K
+// (Lap K = -3 eps^2 s^{-5}). Kept:
+//   * source side, the field (q/6) Lap K of every M2L source (q = sum m |x - z|^2
+//     of the source cell), Taylor-expanded at the target centre to second
+//     order: value into F_0^0, gradient into F_1^m, traceless Hessian part
+//     into F_2^m, its trace into the target-side Laplacian;
+//   * target side, the trace parts of the Taylor series of the cell's own M2L
+//     far field to fourth order,
+//       |d|^2 L/6 + |d|^2 (d.G)/10 + |d|^2 (d.H.d)/28 - |d|^4 tr(H)/280,
+//     from L = Lap Phi, G = grad L, H = Hess L at the centre (10 doubles per
+//     cell). A centre shift would turn part of this polynomial into harmonic
+//     terms, so it is not moved through L2L: L2P sums it over the body's
+//     leaf and every ancestor, each about its own centre.
 // Oracle and device apply the same rule; eps = 0 skips all of it.
 #include "common.h"
 #include "internal.h"
@@ -61,13 +65,15 @@ __global__ void __launch_bounds__(ST) k_trace_up(const uint32_t* __restrict__ ce
 }
 
 // Per target (one wave, lanes striding over its sources): the source-side
-// trace correction of every M2L source into the local expansion (F_0^0,
-// F_1^0, F_1^{+-1}) and the target's Lap Phi.
+// field of every M2L source into the local expansion (F_0^0, F_1^m, F_2^m)
+// and the target-side Laplacian field (L, G, H) at the target centre.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+
+constexpr int NACC = 19;  // F00, F1 (3), F2 (5), L, G (3), H (6)
 
 __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* __restrict__ tptr,
                                                   const uint64_t* __restrict__ skeys, uint64_t smask,
@@ -78,67 +84,75 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
   const uint32_t lane = threadIdx.x & 63;
   if (t >= nc) return;
   const double tx = c.cx[t], ty = c.cy[t], tz = c.cz[t];
-  double f00 = 0.0, f10 = 0.0, f11r = 0.0, f11i = 0.0, L = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  double acc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; k++) acc[k] = 0.0;
   for (uint32_t j = tptr[t] + lane; j < tptr[t + 1]; j += 64) {
     const uint32_t src = (uint32_t)(skeys[j] & smask);
     const double rx = tx - c.cx[src], ry = ty - c.cy[src], rz = tz - c.cz[src];
-    const double s2 = rx * rx + ry * ry + rz * rz + soft2;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double s2 = r2 + soft2;
     const double is = 1.0 / sqrt(s2);
     const double is2 = is * is;
-    const double is5 = is2 * is2 * is;
+    const double is5 = is2 * is2 * is, is7 = is5 * is2, is9 = is7 * is2, is11 = is9 * is2;
+    const double lk = -3.0 * soft2 * is5;                              // Lap K
+    const double gk = 15.0 * soft2 * is7;                              // grad Lap K / r
+    const double hk = 105.0 * soft2 * is9;                             // Hess Lap K, r r^T part
+    const double l4 = 15.0 * soft2 * (3.0 * is7 - 7.0 * r2 * is9);     // Lap^2 K
+    const double g4 = 15.0 * soft2 * (63.0 * r2 * is11 - 35.0 * is9);  // grad Lap^2 K / r
     const double ms = c.mass[src];
-    L -= 3.0 * soft2 * ms * is5;             // Lap of the source monopole's field
-    const double gl = 15.0 * soft2 * ms * is5 / s2;  // its gradient / r
-    gx += gl * rx;
-    gy += gl * ry;
-    gz += gl * rz;
-    const double cc = 0.5 * q[src] * soft2 * is5;  // -(q/6) Lap K at the target centre
-    const double g = 5.0 * cc / s2;                 // its gradient / r
-    f00 -= cc;
-    f10 -= g * rz;
-    f11r -= g * rx;
-    f11i -= g * ry;
+    const double w = q[src] / 6.0;
+    // source side: w Lap K, its gradient, the traceless part of its Hessian
+    const double gw = w * gk, hw = 0.5 * w * hk;
+    acc[0] += w * lk;
+    acc[1] -= gw * rz;                                  // F_1^0
+    acc[2] -= gw * rx;                                  // Re F_1^1
+    acc[3] -= gw * ry;                                  // Im F_1^1
+    acc[4] += 2.0 * hw * (r2 / 3.0 - rz * rz);          // F_2^0
+    acc[5] += -2.0 * hw * rx * rz;                      // Re F_2^1
+    acc[6] += -2.0 * hw * ry * rz;                      // Im F_2^1
+    acc[7] += -2.0 * hw * (rx * rx - ry * ry);          // Re F_2^2
+    acc[8] += -4.0 * hw * rx * ry;                      // Im F_2^2
+    // target side: Laplacian field of the source (monopole + source-side field)
+    acc[9] += ms * lk + w * l4;
+    const double gg = ms * gk + w * g4;
+    acc[10] += gg * rx;
+    acc[11] += gg * ry;
+    acc[12] += gg * rz;
+    const double mh = ms * hk, mg = ms * gk;
+    acc[13] += mg - mh * rx * rx;
+    acc[14] += mg - mh * ry * ry;
+    acc[15] += mg - mh * rz * rz;
+    acc[16] -= mh * rx * ry;
+    acc[17] -= mh * rx * rz;
+    acc[18] -= mh * ry * rz;
   }
-  f00 = wave_sum(f00);
-  f10 = wave_sum(f10);
-  f11r = wave_sum(f11r);
-  f11i = wave_sum(f11i);
-  L = wave_sum(L);
-  gx = wave_sum(gx);
-  gy = wave_sum(gy);
-  gz = wave_sum(gz);
+#pragma unroll
+  for (int k = 0; k < NACC; k++) acc[k] = wave_sum(acc[k]);
   if (lane != 0) return;
   cplx* F = locals + (size_t)t * D;
-  F[0].re += f00;
+  F[0].re += acc[0];
   if (D > 1) {
-    F[2].re += f10;           // (1, 0)
-    F[3].re += f11r;          // (1, 1)
-    F[3].im += f11i;
-    F[1].re -= f11r;          // (1, -1) = -conj(F_1^1)
-    F[1].im += f11i;
+    F[2].re += acc[1];  // (1, 0)
+    F[3].re += acc[2];  // (1, 1)
+    F[3].im += acc[3];
+    F[1].re -= acc[2];  // (1, -1) = -conj(F_1^1)
+    F[1].im += acc[3];
   }
-  lap[4 * (size_t)t] = L;
-  lap[4 * (size_t)t + 1] = gx;
-  lap[4 * (size_t)t + 2] = gy;
-  lap[4 * (size_t)t + 3] = gz;
-}
-
-// Each cell of a level adds its parent's linear Laplacian field shifted to
-// its centre: L += L_p + G_p . (z - z_p), G += G_p (levels top-down).
-__global__ void __launch_bounds__(ST) k_lap_down(const uint32_t* __restrict__ cells, uint32_t cnt,
-                                                DevCells c, double* __restrict__ lap) {
-  const uint32_t i = blockIdx.x * ST + threadIdx.x;
-  if (i >= cnt) return;
-  const uint32_t id = cells[i];
-  const int32_t pa = c.parent[id];
-  if (pa < 0) return;
-  const double* lp = lap + 4 * (size_t)pa;
-  double* lc = lap + 4 * (size_t)id;
-  lc[0] += lp[0] + lp[1] * (c.cx[id] - c.cx[pa]) + lp[2] * (c.cy[id] - c.cy[pa]) +
-           lp[3] * (c.cz[id] - c.cz[pa]);
-  lc[1] += lp[1];
-  lc[2] += lp[2];
-  lc[3] += lp[3];
+  if (D > 4) {
+    F[6].re += acc[4];  // (2, 0)
+    F[7].re += acc[5];  // (2, 1)
+    F[7].im += acc[6];
+    F[5].re -= acc[5];  // (2, -1) = -conj(F_2^1)
+    F[5].im += acc[6];
+    F[8].re += acc[7];  // (2, 2)
+    F[8].im += acc[8];
+    F[4].re += acc[7];  // (2, -2) = conj(F_2^2)
+    F[4].im -= acc[8];
+  }
+  double* o = lap + (size_t)SOFT_LAP_WORDS * t;
+#pragma unroll
+  for (int k = 0; k < SOFT_LAP_WORDS; k++) o[k] = acc[9 + k];
 }
 
 void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
@@ -161,11 +175,6 @@ void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, 
   const uint64_t smask = (1ull << sbits) - 1ull;
   hipLaunchKernelGGL(k_m2l_trace, dim3((nc + 3) / 4), dim3(256), 0, s, nc, tptr, skeys, smask, c,
                      q, soft2, hdim(p), locals, lap);
-}
-
-void launch_lap_down(const uint32_t* cells, uint32_t cnt, DevCells c, double* lap, hipStream_t s) {
-  if (cnt == 0) return;
-  hipLaunchKernelGGL(k_lap_down, dim3((cnt + ST - 1) / ST), dim3(ST), 0, s, cells, cnt, c, lap);
 }
 
 }  // namespace fmmh

@@ -706,6 +706,7 @@ struct orc_ctx {
   int p = 0;
   std::vector<C> moments, locals_walk, locals;
   std::vector<double> pows, f, fmin, lap;
+  std::vector<int32_t> parent;  // softened far field: L2P walks the ancestors
   std::vector<uint32_t> m2l, p2p;
   Bodies bodies;
   double times[5] = {0, 0, 0, 0, 0};
@@ -1087,7 +1088,7 @@ void orc_binomials(int p, double* out) {
 
 static void trace_pair(int p, const double* zs, const double* zt, double q, double mass,
                        double soft2, C* f, double* lap);
-static void l2p_trace(const double* cen, const double* x, const double* L4, double* pot,
+static void l2p_trace(const double* cen, const double* x, const double* L10, double* pot,
                       double* a);
 
 void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], const double* ma,
@@ -1097,13 +1098,13 @@ void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], co
 }
 
 void orc_soft_trace_pair(int p, const double cen_src[3], const double cen_dst[3], double q,
-                         double mass, double eps2, double* f_dst, double* lap4_dst) {
-  trace_pair(p, cen_src, cen_dst, q, mass, eps2, (C*)f_dst, lap4_dst);
+                         double mass, double eps2, double* f_dst, double* lap10_dst) {
+  trace_pair(p, cen_src, cen_dst, q, mass, eps2, (C*)f_dst, lap10_dst);
 }
 
-void orc_l2p_trace(const double cen[3], const double pos[3], const double* lap4, double* pot,
+void orc_l2p_trace(const double cen[3], const double pos[3], const double* lap10, double* pot,
                    double acc[3]) {
-  l2p_trace(cen, pos, lap4, pot, acc);
+  l2p_trace(cen, pos, lap10, pot, acc);
 }
 
 void orc_m2l_mutual(int p, const double cen_a[3], const double cen_b[3], const double* ma,
@@ -1270,51 +1271,103 @@ void orc_get_bbox(const orc_ctx* t, double lo[3], double hi[3], double* inv_side
 // Softening extension, far field: the Taylor expansions of the Plummer
 // kernel K = (r^2 + eps^2)^{-1/2} about the source and the target centre
 // have, beside the harmonic (solid-harmonic) parts that M2L and L2P carry,
-// trace terms |d|^{2k} Y_{n-2k}(d) Lap^k K. The leading ones are kept:
-//  * source side, (q/6) Lap K with q = sum m |x - z|^2 and
-//    Lap K = -3 eps^2 s^{-5}: added to every M2L target's local expansion to
-//    first order (value into F_0^0, gradient into F_1^m; l2p_into gives
-//    pot = F00 - F10 dz - Re F11 dx - Im F11 dy);
-//  * target side, (|d|^2 / 6) Lap Phi + (|d|^2 / 10) d.grad(Lap Phi): per
-//    cell the Laplacian of its M2L sources' monopole fields and its gradient
-//    at the centre (lap[4c..4c+3]), Taylor-shifted to the children (L2L) and
-//    applied at L2P.
-// q is summed directly from the bodies of each cell.
-// One directed M2L pair's trace terms: source-side correction into the
-// target's local expansion f, Lap of the source's monopole field into *lap.
+// trace terms |d|^{2k} Y_{n-2k}(d) Lap^k K (Lap K = -3 eps^2 s^{-5}). Kept:
+//  * source side, the field (q/6) Lap K of every M2L source (q = sum m |x - z|^2,
+//    summed from the bodies), Taylor-expanded at the target centre to second
+//    order: value into F_0^0, gradient into F_1^m, the traceless part of
+//    the Hessian into F_2^m, its trace into the target-side Laplacian below
+//    (l2p_into gives pot = F00 - F10 dz - Re F11 dx - Im F11 dy + ...);
+//  * target side, the trace parts of the Taylor series of the cell's own
+//    M2L far field to fourth order,
+//      |d|^2 L / 6 + |d|^2 (d.G) / 10 + |d|^2 (d.H.d) / 28 - |d|^4 tr(H) / 280,
+//    from L = Lap Phi, G = grad L and H = Hess L at the centre (the monopole
+//    fields of the sources plus the source-side fields above; 10 doubles per
+//    cell). Shifting a centre turns part of this polynomial into harmonic
+//    terms, so it is not moved through L2L: each body sums it over its leaf
+//    and every ancestor, each about its own centre (the exact re-expansion).
+// Derivatives used, with s^2 = r^2 + eps^2:
+//   grad Lap K = 15 eps^2 s^-7 r;  Hess Lap K = 15 eps^2 (s^-7 I - 7 s^-9 r r^T);
+//   Lap^2 K = 15 eps^2 (3 s^-7 - 7 r^2 s^-9);
+//   grad Lap^2 K = 15 eps^2 (63 r^2 s^-11 - 35 s^-9) r.
+constexpr int SOFT_LAP_WORDS = 10;  // L, G (3), H (xx, yy, zz, xy, xz, yz)
+
+// One directed M2L pair's trace terms: source-side field into the target's
+// local expansion f, the target-side Laplacian field into lap[0..9].
 static void trace_pair(int p, const double* zs, const double* zt, double q, double mass,
                        double soft2, C* f, double* lap) {
   double r[3] = {zt[0] - zs[0], zt[1] - zs[1], zt[2] - zs[2]};
-  double s2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + soft2;
+  double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  double s2 = r2 + soft2;
   double is = 1.0 / std::sqrt(s2);
-  double is5 = is * is * is * is * is;
-  lap[0] += -3.0 * soft2 * mass * is5;  // Lap of the source's monopole field
-  double gl = 15.0 * soft2 * mass * is5 / s2;  // and its gradient / r
-  lap[1] += gl * r[0];
-  lap[2] += gl * r[1];
-  lap[3] += gl * r[2];
-  double c = 0.5 * q * soft2 * is5;  // -(q/6) Lap K at zt
-  double g[3] = {5.0 * c * r[0] / s2, 5.0 * c * r[1] / s2, 5.0 * c * r[2] / s2};  // its gradient
-  f[idx(0, 0)].re -= c;
+  double is2 = is * is;
+  double is5 = is2 * is2 * is, is7 = is5 * is2, is9 = is7 * is2, is11 = is9 * is2;
+  double lk = -3.0 * soft2 * is5;                          // Lap K
+  double gk = 15.0 * soft2 * is7;                          // grad Lap K / r (and H's I part)
+  double hk = 105.0 * soft2 * is9;                         // H's r r^T part
+  double l4 = 15.0 * soft2 * (3.0 * is7 - 7.0 * r2 * is9);  // Lap^2 K
+  double g4 = 15.0 * soft2 * (63.0 * r2 * is11 - 35.0 * is9);  // grad Lap^2 K / r
+  double w = q / 6.0;
+  // target side: Laplacian field of the source (monopole + source-side field)
+  lap[0] += mass * lk + w * l4;
+  double gg = mass * gk + w * g4;
+  lap[1] += gg * r[0];
+  lap[2] += gg * r[1];
+  lap[3] += gg * r[2];
+  double mh = mass * hk;
+  lap[4] += mass * gk - mh * r[0] * r[0];
+  lap[5] += mass * gk - mh * r[1] * r[1];
+  lap[6] += mass * gk - mh * r[2] * r[2];
+  lap[7] -= mh * r[0] * r[1];
+  lap[8] -= mh * r[0] * r[2];
+  lap[9] -= mh * r[1] * r[2];
+  // source side: w Lap K, its gradient, and the traceless part of its Hessian
+  // (w/2) d^T (Hess Lap K - tr/3) d = -(w/2) hk (d.r)^2 + (w/2) hk r^2 |d|^2 / 3
+  double v = w * lk, gw = w * gk;
+  f[idx(0, 0)].re += v;
   if (p >= 1) {
-    f[idx(1, 0)].re -= g[2];
-    f[idx(1, 1)].re -= g[0];
-    f[idx(1, 1)].im -= g[1];
-    f[idx(1, -1)].re += g[0];
-    f[idx(1, -1)].im -= g[1];
+    f[idx(1, 0)].re -= gw * r[2];
+    f[idx(1, 1)].re -= gw * r[0];
+    f[idx(1, 1)].im -= gw * r[1];
+    f[idx(1, -1)].re += gw * r[0];
+    f[idx(1, -1)].im -= gw * r[1];
+  }
+  if (p >= 2) {
+    double hw = 0.5 * w * hk;
+    double e = 2.0 * hw * (r2 / 3.0 - r[2] * r[2]);   // 2 S_zz
+    double ar = -2.0 * hw * r[0] * r[2];             // 2 S_xz
+    double ai = -2.0 * hw * r[1] * r[2];             // 2 S_yz
+    double br = -2.0 * hw * (r[0] * r[0] - r[1] * r[1]);  // 2 (S_xx - S_yy)
+    double bi = -4.0 * hw * r[0] * r[1];             // 4 S_xy
+    f[idx(2, 0)].re += e;
+    f[idx(2, 1)].re += ar;
+    f[idx(2, 1)].im += ai;
+    f[idx(2, -1)].re -= ar;
+    f[idx(2, -1)].im += ai;
+    f[idx(2, 2)].re += br;
+    f[idx(2, 2)].im += bi;
+    f[idx(2, -2)].re += br;
+    f[idx(2, -2)].im -= bi;
   }
 }
 
-// Target-side trace terms at L2P, from the far field's Laplacian L and its
-// gradient G at the cell centre: the second- and third-order trace parts of
-// the Taylor series, pot += |d|^2 L / 6 + |d|^2 (d.G) / 10, a = their gradient.
-static void l2p_trace(const double* cen, const double* x, const double* L4, double* pot,
+// Target-side trace polynomial of one cell at L2P (d = x - cen):
+// pot += |d|^2 L/6 + |d|^2 (d.G)/10 + |d|^2 (d.H.d)/28 - |d|^4 tr(H)/280,
+// a += its gradient.
+static void l2p_trace(const double* cen, const double* x, const double* L10, double* pot,
                       double* a) {
   double d[3] = {x[0] - cen[0], x[1] - cen[1], x[2] - cen[2]};
   double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  double L = L4[0], dg = d[0] * L4[1] + d[1] * L4[2] + d[2] * L4[3];
-  *pot += d2 * L / 6.0 + d2 * dg / 10.0;
-  for (int k = 0; k < 3; k++) a[k] += d[k] * L / 3.0 + (2.0 * d[k] * dg + d2 * L4[1 + k]) / 10.0;
+  double L = L10[0];
+  double dg = d[0] * L10[1] + d[1] * L10[2] + d[2] * L10[3];
+  double hd[3] = {L10[4] * d[0] + L10[7] * d[1] + L10[8] * d[2],
+                  L10[7] * d[0] + L10[5] * d[1] + L10[9] * d[2],
+                  L10[8] * d[0] + L10[9] * d[1] + L10[6] * d[2]};
+  double dhd = d[0] * hd[0] + d[1] * hd[1] + d[2] * hd[2];
+  double tr = L10[4] + L10[5] + L10[6];
+  *pot += d2 * L / 6.0 + d2 * dg / 10.0 + d2 * dhd / 28.0 - d2 * d2 * tr / 280.0;
+  for (int k = 0; k < 3; k++)
+    a[k] += d[k] * L / 3.0 + (2.0 * d[k] * dg + d2 * L10[1 + k]) / 10.0 +
+            (d[k] * dhd + d2 * hd[k]) / 14.0 - d2 * d[k] * tr / 70.0;
 }
 
 static void softened_trace(orc_ctx& t, int p, double soft2) {
@@ -1330,15 +1383,19 @@ static void softened_trace(orc_ctx& t, int p, double soft2) {
     }
     q[c] = acc;
   }
-  t.lap.assign(4 * nc, 0.0);  // per cell: Lap Phi and its gradient at the centre
+  t.lap.assign(SOFT_LAP_WORDS * nc, 0.0);  // per cell: L, G, H of its own M2L sources
   auto add = [&](size_t src, size_t dst) {
     trace_pair(p, t.cells[src].cen, t.cells[dst].cen, q[src], t.cells[src].mass, soft2,
-               t.locals.data() + dst * d, &t.lap[4 * dst]);
+               t.locals.data() + dst * d, &t.lap[SOFT_LAP_WORDS * dst]);
   };
   for (size_t k = 0; k + 1 < t.m2l.size(); k += 2) {
     add(t.m2l[k], t.m2l[k + 1]);
     add(t.m2l[k + 1], t.m2l[k]);
   }
+  t.parent.assign(nc, -1);
+  for (size_t c = 0; c < nc; c++)
+    for (int k = 0; k < 8; k++)
+      if (t.cells[c].children[k] >= 0) t.parent[t.cells[c].children[k]] = (int32_t)c;
 }
 
 // fmm.rs:69-142
@@ -1552,16 +1609,6 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
           if (ch < 0) continue;
           l2l_into(p, cell.cen, t->cells[ch].cen, t->locals.data() + c * d,
                    t->locals.data() + (size_t)ch * d, u.data());
-          if (!t->lap.empty()) {  // shift the parent's linear Laplacian field to the child
-            const double* lp = &t->lap[4 * c];
-            double* lc = &t->lap[4 * (size_t)ch];
-            const double* zc = t->cells[ch].cen;
-            lc[0] += lp[0] + lp[1] * (zc[0] - cell.cen[0]) + lp[2] * (zc[1] - cell.cen[1]) +
-                     lp[3] * (zc[2] - cell.cen[2]);
-            lc[1] += lp[1];
-            lc[2] += lp[2];
-            lc[3] += lp[3];
-          }
         }
       }
     });
@@ -1581,7 +1628,9 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
         double a[3] = {0.0, 0.0, 0.0};
         double x[3] = {t->bodies.x[k], t->bodies.y[k], t->bodies.z[k]};
         l2p_into(p, cell.cen, x, t->locals.data() + c * d, &p0, a, u.data());
-        if (!t->lap.empty()) l2p_trace(cell.cen, x, &t->lap[4 * c], &p0, a);
+        if (!t->lap.empty())  // the leaf's and every ancestor's trace polynomial
+          for (int32_t A = (int32_t)c; A >= 0; A = t->parent[A])
+            l2p_trace(t->cells[A].cen, x, &t->lap[SOFT_LAP_WORDS * (size_t)A], &p0, a);
         pot[id] += t->bodies.pot[k] + p0;
         accel[3 * id + 0] += t->bodies.ax[k] + a[0];
         accel[3 * id + 1] += t->bodies.ay[k] + a[1];

@@ -63,16 +63,17 @@ void orc_binomials(int p, double* out /*p+1*/);
 void orc_m2l_mutual(int p, const double cen_a[3], const double cen_b[3], const double* ma,
                     const double* mb, double* la, double* lb);
 /* Softening extension of the far field: mutual M2L with the harmonic
- * derivatives of the Plummer kernel, one directed pair's trace correction
- * (source side into f_dst; the Laplacian of the source monopole's field and
- * its gradient at the target centre added into lap4_dst[0..3]) and the
- * target-side L2P terms (pot += |d|^2 L / 6 + |d|^2 (d.G) / 10, acc = their
- * gradient). */
+ * derivatives of the Plummer kernel; one directed pair's trace terms (the
+ * source-side field (q/6) Lap K to second order into f_dst, the target-side
+ * Laplacian field of the source (L, grad L, Hessian of L: 10 doubles, see
+ * SOFT_LAP_WORDS) added into lap10_dst); and the target-side L2P polynomial of
+ * one cell's lap10 (pot += |d|^2 L / 6 + |d|^2 (d.G) / 10 + |d|^2 (d.H.d) / 28
+ * - |d|^4 tr(H) / 280, acc = its gradient). */
 void orc_m2l_mutual_soft(int p, const double cen_a[3], const double cen_b[3], const double* ma,
                          const double* mb, double eps2, double* la, double* lb);
 void orc_soft_trace_pair(int p, const double cen_src[3], const double cen_dst[3], double q,
-                         double mass, double eps2, double* f_dst, double* lap4_dst);
-void orc_l2p_trace(const double cen[3], const double pos[3], const double* lap4, double* pot,
+                         double mass, double eps2, double* f_dst, double* lap10_dst);
+void orc_l2p_trace(const double cen[3], const double pos[3], const double* lap10, double* pot,
                    double acc[3]);
 /* rotation.rs:262-435 mutual rotation M2L on explicit centres/moments */
 void orc_m2l_rotation(int p, const double cen_a[3], const double cen_b[3], const double* ma,

@@ -271,20 +271,28 @@ def m2l_mutual_soft(p, cen_a, cen_b, ma, mb, eps2):
     return la, lb
 
 
+SOFT_LAP_WORDS = 10  # L, G (3), H (xx, yy, zz, xy, xz, yz)
+
+
 def soft_trace_pair(p, cen_src, cen_dst, q, mass, eps2, f_dst):
-    """Adds the source-side trace correction into f_dst (in place); returns
-    [Lap, grad Lap] of the source monopole's field at the target centre."""
-    lap = np.zeros(4)
+    """Adds the source-side field (q/6) Lap K to second order into f_dst (in
+    place); returns the target-side Laplacian field of the source at the
+    target centre: [L, G (3), H (xx, yy, zz, xy, xz, yz)]."""
+    assert f_dst.dtype == np.complex128 and f_dst.flags.c_contiguous and f_dst.size >= dim(p)
+    lap = np.zeros(SOFT_LAP_WORDS)
     a, b = _f64(cen_src), _f64(cen_dst)
     lib().orc_soft_trace_pair(p, _p(a), _p(b), float(q), float(mass), float(eps2),
                               f_dst.ctypes.data_as(_pd), _p(lap))
     return lap
 
 
-def l2p_trace(cen, pos, lap4):
+def l2p_trace(cen, pos, lap10):
+    """Target-side trace polynomial of one cell's [L, G, H] at pos."""
+    lap10 = _f64(lap10)
+    assert lap10.shape == (SOFT_LAP_WORDS,)
     pot = np.zeros(1)
     acc = np.zeros(3)
-    lib().orc_l2p_trace(_p(_f64(cen)), _p(_f64(pos)), _p(_f64(lap4)), _p(pot), _p(acc))
+    lib().orc_l2p_trace(_p(_f64(cen)), _p(_f64(pos)), _p(lap10), _p(pot), _p(acc))
     return pot[0], acc
 
 

@@ -491,14 +491,62 @@ def _cluster(rng, nb, rad):
     return d, m
 
 
+def _laplacian_field_exact(eps):
+    """L = Lap K, G = grad L, H = Hess L of the Plummer kernel K, by computer
+    algebra (independent of the oracle's closed forms)."""
+    sp = pytest.importorskip("sympy")
+    x, y, z = sp.symbols("x y z", real=True)
+    K = 1 / sp.sqrt(x * x + y * y + z * z + eps * eps)
+    L = sp.diff(K, x, 2) + sp.diff(K, y, 2) + sp.diff(K, z, 2)
+    G = [sp.diff(L, v) for v in (x, y, z)]
+    H = [sp.diff(L, x, 2), sp.diff(L, y, 2), sp.diff(L, z, 2), sp.diff(L, x, y),
+         sp.diff(L, x, z), sp.diff(L, y, z)]
+    f = sp.lambdify((x, y, z), [L] + G + H, "math")
+    return lambda r: np.array(f(*r), float)
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.2])
+def test_softened_laplacian_field_against_computer_algebra(eps):
+    """The target-side Laplacian field of one M2L source, [L, G, H] at the
+    target centre, against the exact sum over the source's bodies of Lap K,
+    grad Lap K and Hess Lap K (computer algebra). A point mass matches to
+    rounding; a small octahedral cluster (radius a, no dipole, no
+    quadrupole, no third moments) matches L and G to O(a^4) because the
+    source-side field (q/6) Lap^2 K is in them; leaving it out (a plausible
+    omission) misses by O(a^2) and fails the same bound."""
+    fld = _laplacian_field_exact(eps)
+    zA = np.zeros(3)
+    u = np.array([0.3, -0.5, 0.8])
+    zB = u / np.linalg.norm(u)
+    zero = np.zeros(O.dim(4), np.complex128)
+    lap = O.soft_trace_pair(4, zA, zB, 0.0, 2.5, eps * eps, zero.copy())
+    want = 2.5 * fld(zB - zA)
+    assert np.allclose(lap, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+    a = 0.02
+    d = a * np.vstack([np.eye(3), -np.eye(3)])
+    m = np.full(6, 0.5)
+    q = float(np.sum(m * (d * d).sum(1)))
+    lap = O.soft_trace_pair(4, zA, zB, q, m.sum(), eps * eps, zero.copy())
+    want = sum(mi * fld(zB - (zA + di)) for mi, di in zip(m, d))
+    lap0 = O.soft_trace_pair(4, zA, zB, 0.0, m.sum(), eps * eps, zero.copy())
+    for sl, tol in ((slice(0, 1), 1e-5), (slice(1, 4), 1e-5), (slice(4, 10), 5e-3)):
+        scale = np.abs(want[sl]).max()
+        assert np.abs(lap[sl] - want[sl]).max() < tol * scale, sl
+    for sl in (slice(0, 1), slice(1, 4)):
+        scale = np.abs(want[sl]).max()
+        assert np.abs(lap0[sl] - want[sl]).max() > 10 * 1e-5 * scale, sl
+
+
 @pytest.mark.parametrize("eps", [0.02, 0.05])
 def test_softened_far_field_against_exact_sum(eps):
     """One well-separated pair of clusters (radii 0.15, separation 1, p = 8):
     M2L with the harmonic derivatives of the Plummer kernel, the source-side
-    trace correction, L2P and the target-side trace term, against the exact
-    softened sum. Dropping the trace terms (a plausible omission) leaves the
-    leading O(eps^2 rho / r^3) error and fails the same bound; eps -> 0
-    reproduces the unsoftened M2L bit for bit."""
+    field (q/6) Lap K to second order, L2P and the target-side trace
+    polynomial to fourth order, against the exact softened sum. Each
+    plausible omission -- no trace terms at all, no second-order (F_2) part
+    of the source-side field, no Hessian H, no source-side part of L and G --
+    fails the same bound; eps -> 0 reproduces the unsoftened M2L bit for
+    bit."""
     p = 8
     rng = np.random.default_rng(11)
     dA, mA = _cluster(rng, 300, 0.15)
@@ -512,31 +560,67 @@ def test_softened_far_field_against_exact_sum(eps):
     _, F = O.m2l_mutual_soft(p, zA, zB, M, zero, eps * eps)
     Fc = F.copy()
     lap = O.soft_trace_pair(p, zA, zB, q, mA.sum(), eps * eps, Fc)
-    s2 = 1 + eps * eps
-    assert lap[0] == pytest.approx(-3 * eps * eps * mA.sum() * s2 ** -2.5, rel=1e-14)
-    assert np.allclose(lap[1:], 15 * eps * eps * mA.sum() * s2 ** -3.5 * zB, rtol=1e-14)
-    err_with, err_without = [], []
-    for db in dB:
-        x = zB + db
-        diff = x - (zA + dA)
-        s = np.sqrt((diff ** 2).sum(1) + eps * eps)
-        a_ex = -(mA[:, None] * diff / s[:, None] ** 3).sum(0)
-        p_ex = np.sum(mA / s)
-        p0, a0 = O.l2p(p, zB, x, F)
-        p1, a1 = O.l2p(p, zB, x, Fc)
-        pt, at = O.l2p_trace(zB, x, lap)
-        p1, a1 = p1 + pt, a1 + at
-        err_with.append(max(np.linalg.norm(a1 - a_ex) / np.linalg.norm(a_ex),
+    corr = Fc - F
+    lap_noq = O.soft_trace_pair(p, zA, zB, 0.0, mA.sum(), eps * eps, zero.copy())
+
+    def median_err(Fx, lx, with_trace=True):
+        errs = []
+        for db in dB:
+            x = zB + db
+            diff = x - (zA + dA)
+            s = np.sqrt((diff ** 2).sum(1) + eps * eps)
+            a_ex = -(mA[:, None] * diff / s[:, None] ** 3).sum(0)
+            p_ex = np.sum(mA / s)
+            p1, a1 = O.l2p(p, zB, x, Fx)
+            if with_trace:
+                pt, at = O.l2p_trace(zB, x, lx)
+                p1, a1 = p1 + pt, a1 + at
+            errs.append(max(np.linalg.norm(a1 - a_ex) / np.linalg.norm(a_ex),
                             abs(p1 - p_ex) / p_ex))
-        err_without.append(np.linalg.norm(a0 - a_ex) / np.linalg.norm(a_ex))
-    bound = 0.6 * (eps / 0.05) ** 2 * 2e-4
-    assert np.median(err_with) < bound, (np.median(err_with), bound)
-    assert np.median(err_without) > bound
-    assert np.median(err_with) < 0.5 * np.median(err_without)
+        return np.median(errs)
+
+    bound = 1.4e-5 * (eps / 0.05) ** 2
+    assert median_err(Fc, lap) < bound
+    assert median_err(F, lap, with_trace=False) > 5 * bound  # no trace terms
+    no_f2 = F + np.where(np.arange(O.dim(p)) >= 4, 0, corr)
+    assert median_err(no_f2, lap) > bound  # source-side field to first order only
+    no_h = lap.copy()
+    no_h[4:] = 0.0
+    assert median_err(Fc, no_h) > bound  # target-side polynomial to third order only
+    no_ql = lap.copy()
+    no_ql[:4] = lap_noq[:4]
+    assert median_err(Fc, no_ql) > bound  # source-side field missing from L, G
     # eps -> 0: the unsoftened M2L exactly
     _, F0 = O.m2l_mutual_soft(p, zA, zB, M, zero, 0.0)
     _, Fu = O.m2l_mutual(p, zA, zB, M, zero)
     assert np.array_equal(F0, Fu)
+
+
+def test_softened_core_error_floor():
+    """Full oracle FMM where softening reaches into the far field: Plummer
+    N = 262,144, eps = 4e-3, p = 10, theta = 0.5, n_leaf = 128 (eps relative
+    to the core's leaf size as eps = 1e-3 at 16.7M bodies). Thousands of M2L
+    pairs per core target add their softening residuals coherently; with
+    the trace terms to the orders kept here the 99th-percentile error stays
+    within 1.5x of the unsoftened run's. Moving the target-side polynomial
+    through L2L as a Laplacian field, without the harmonic terms that a
+    centre shift creates, leaves it 8x above; stopping the polynomial at
+    third order, 4x."""
+    n = 262144
+    pos, _, mass = O.ic_plummer(n, seed=42)
+    ids = O.sample_ids(n, 2048, 7)
+    p99 = {}
+    for soft in (0.0, 4e-3):
+        acc = np.empty((len(ids), 3))
+        pot = np.empty(len(ids))
+        ff = np.empty(len(ids))
+        O.lib().orc_direct_targets(n, O._p(pos), O._p(mass), soft, len(ids),
+                                   O._p(ids, O._pu32), O._p(acc), O._p(pot), O._p(ff), 8)
+        f = O.OracleFmm(pos, mass, p=10, eps=math.inf, n_leaf=128, theta_max=0.5, softening=soft)
+        a, _ = f.run(pos, mass, nthreads=8)
+        rel = np.linalg.norm(a[ids] - acc, axis=1) / np.linalg.norm(acc, axis=1)
+        p99[soft] = np.percentile(rel, 99)
+    assert p99[4e-3] < 1.5 * p99[0.0], p99
 
 
 def test_softened_fmm_error_matches_unsoftened_truncation():
