@@ -58,7 +58,8 @@ class FmmError(RuntimeError):
 class _Params(ct.Structure):
     _fields_ = [("p", ct.c_int), ("eps_mac", ct.c_double), ("theta_max", ct.c_double),
                 ("n_leaf", ct.c_int), ("softening", ct.c_double), ("center_mode", ct.c_int),
-                ("precision", ct.c_int), ("flags", ct.c_uint), ("hbm_budget_bytes", ct.c_double)]
+                ("precision", ct.c_int), ("flags", ct.c_uint), ("hbm_budget_bytes", ct.c_double),
+                ("soft_mac", ct.c_double)]
 
 
 class _LeapOpts(ct.Structure):
@@ -169,16 +170,20 @@ class Fmm:
 
     Extensions beyond the reference (defaults reproduce it): theta_max
     (opening-angle bound, 1.0 = theta < 1), softening (Plummer softening of
-    the near field, 0 = pure 1/r), eps = inf for the theta-only MAC."""
+    the kernel, near and far field; 0 = pure 1/r), soft_mac (softening guard
+    of the MAC: M2L only where softening^2 (rhoA + rhoB)^3 <= soft_mac r^5;
+    0 = off), eps = inf for the theta-only MAC."""
 
     def __init__(self, pos, mass, p, eps, n_leaf, center_mode=CenterMode.Com, theta_max=1.0,
-                 softening=0.0, timing=False, hbm_budget_bytes=0.0, precision="fp64"):
+                 softening=0.0, timing=False, hbm_budget_bytes=0.0, precision="fp64",
+                 soft_mac=0.0):
         prm = _Params()
         lib().fmmh_default_params(ct.byref(prm))
         prm.p, prm.eps_mac, prm.theta_max, prm.n_leaf = int(p), float(eps), float(theta_max), int(n_leaf)
         prm.softening, prm.center_mode = float(softening), int(center_mode)
         prm.flags = FMMH_FLAG_TIMING if timing else 0
         prm.hbm_budget_bytes = float(hbm_budget_bytes)
+        prm.soft_mac = float(soft_mac)
         prm.precision = {"fp64": FMMH_PREC_FP64, "fp32bulk": FMMH_PREC_FP32BULK}[precision]
         self.precision = precision
         self.p, self.eps, self.n_leaf = int(p), float(eps), int(n_leaf)

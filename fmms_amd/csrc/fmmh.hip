@@ -702,6 +702,8 @@ struct fmmh_ctx {
     mp.pows = (const double*)pows.p;
     mp.binom = (const double*)binom.p;
     mp.fmin = (const double*)fmin.p;
+    mp.soft2 = prm.softening * prm.softening;
+    mp.soft_mac = prm.soft_mac;
     // the walk starts from the self pair of every (system) root: level vlev
     uint32_t nf = lvl_off[vlev + 1] - lvl_off[vlev];
     launch_front_init((const uint32_t*)lvl_cells.p + lvl_off[vlev], nf, front_a.p, s);
@@ -765,6 +767,10 @@ int check_params(const fmmh_params* p, std::string* why) {
     *why = "softening must be finite and >= 0";
     return FMMH_E_PARAM;
   }
+  if (!(p->soft_mac >= 0.0) || !std::isfinite(p->soft_mac)) {
+    *why = "soft_mac must be finite and >= 0";
+    return FMMH_E_PARAM;
+  }
   if (p->center_mode != FMMH_CENTER_COM && p->center_mode != FMMH_CENTER_SPHERE) {
     *why = "center_mode must be FMMH_CENTER_COM or FMMH_CENTER_SPHERE";
     return FMMH_E_PARAM;
@@ -824,6 +830,7 @@ void fmmh_default_params(fmmh_params* out) {
   out->precision = FMMH_PREC_FP64;
   out->flags = 0;
   out->hbm_budget_bytes = 0.0;
+  out->soft_mac = 0.0;
 }
 
 int fmmh_create(const fmmh_params* params, fmmh_ctx** out) {

@@ -110,6 +110,8 @@ struct MacParams {
   const double* pows;
   const double* binom;
   const double* fmin;
+  double soft2;     // softening^2
+  double soft_mac;  // softening guard tau (0 = off)
 };
 size_t walk_rec_bytes();
 void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s);

@@ -77,11 +77,16 @@ __device__ __forceinline__ double error_estimate(uint32_t a, double rho_b, doubl
 
 // mac.rs:64-95 for both directions of one pair. theta_max extension:
 // r2*theta_max^2 <= (rhoA+rhoB)^2 rejects; theta_max = 1 multiplies by 1.
+// Softening guard extension: eps^2 (rhoA+rhoB)^3 > tau r^5 rejects (the
+// softened far field's leading residual relative to the pair's field).
 __device__ __forceinline__ bool accept(uint32_t a, const WalkRec& ra, uint32_t b,
                                        const WalkRec& rb, double r2, const double* __restrict__ mass,
                                        const MacParams& mp) {
   double sum_rho = ra.rho + rb.rho;
   if (r2 * mp.theta2 <= sum_rho * sum_rho) return false;
+  if (mp.soft_mac > 0.0 && mp.soft2 > 0.0 &&
+      mp.soft2 * (sum_rho * sum_rho * sum_rho) > mp.soft_mac * (r2 * r2 * sqrt(r2)))
+    return false;
   if (!mp.use_err) return true;
   double r = sqrt(r2);
   if (r == 0.0) return false;

@@ -61,11 +61,16 @@ typedef struct fmmh_params {
                        +INFINITY = theta-only MAC, no f pre-pass */
   double theta_max; /* opening angle bound; 1.0 = the reference's hard-coded theta<1 */
   int n_leaf;       /* leaf capacity (Fmm.n_leaf) */
-  double softening; /* Plummer softening of the near field; 0 = the reference (pure 1/r) */
+  double softening; /* Plummer softening of the kernel (near field exact, far field through
+                       the softened expansions); 0 = the reference (pure 1/r) */
   int center_mode;  /* FMMH_CENTER_* */
   int precision;    /* FMMH_PREC_* */
   unsigned flags;   /* FMMH_FLAG_* */
   double hbm_budget_bytes; /* device-memory budget of this context; 0 = no limit */
+  double soft_mac;  /* softening guard of the MAC (extension, softening > 0 only): a pair
+                       is rejected for M2L when softening^2 (rhoA + rhoB)^3 > soft_mac r^5,
+                       the softened far field's leading residual relative to the pair's
+                       field; 0 = off (default) */
 } fmmh_params;
 
 #define FMMH_MAX_P 40

@@ -707,6 +707,7 @@ struct orc_ctx {
   std::vector<C> moments, locals_walk, locals;
   std::vector<double> pows, f, fmin, lap;
   std::vector<int32_t> parent;  // softened far field: L2P walks the ancestors
+  double soft_mac = 0.0;        // softening guard of the MAC (orc_set_soft_mac)
   std::vector<uint32_t> m2l, p2p;
   Bodies bodies;
   double times[5] = {0, 0, 0, 0, 0};
@@ -937,6 +938,8 @@ struct Mac {
   const double* pows;
   const double* binom;
   const double* fmin;
+  double soft2;     // softening^2
+  double soft_mac;  // softening guard tau (0 = off)
 
   // mac.rs:23-61
   bool error_estimate(size_t a, const Cell& cb, double ca_mass, double ca_rho, double r,
@@ -970,6 +973,11 @@ struct Mac {
     double r2 = dx * dx + dy * dy + dz * dz;
     double sum_rho = ca.rho + cb.rho;
     if (r2 * theta2 <= sum_rho * sum_rho) return false;
+    // softening guard (extension): the softened far field's leading residual,
+    // eps^2 (rhoA + rhoB)^3 / r^5 relative to the pair's field, above tau rejects
+    if (soft_mac > 0.0 && soft2 > 0.0 &&
+        soft2 * (sum_rho * sum_rho * sum_rho) > soft_mac * (r2 * r2 * std::sqrt(r2)))
+      return false;
     if (!use_err) return true;
     double r = std::sqrt(r2);
     double est;
@@ -1239,6 +1247,7 @@ int orc_build(orc_ctx* t, size_t n, const double* pos, const double* mass, int n
 }
 
 size_t orc_ncells(const orc_ctx* t) { return t->cells.size(); }
+void orc_set_soft_mac(orc_ctx* t, double tau) { t->soft_mac = tau; }
 uint32_t orc_max_level(const orc_ctx* t) { return t->max_level; }
 
 void orc_get_tree(const orc_ctx* t, uint32_t* order, uint64_t* keys_sorted, uint32_t* first,
@@ -1437,7 +1446,8 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
     t->fmin.assign(nc, INFINITY);
   }
   double t2 = now_ms();
-  Mac mac{t, p, eps_mac, theta_max * theta_max, use_err, t->pows.data(), binom.data(), t->fmin.data()};
+  Mac mac{t,           p,           eps_mac,         theta_max * theta_max, use_err,
+          t->pows.data(), binom.data(), t->fmin.data(), soft2,                 t->soft_mac};
   t->locals.assign(nc * d, cz());
   t->m2l.clear();
   t->p2p.clear();

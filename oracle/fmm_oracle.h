@@ -113,6 +113,10 @@ void orc_get_bbox(const orc_ctx*, double lo[3], double hi[3], double* inv_side);
  * nthreads>1 : run_threaded analogue (level-parallel passes, recorded walk,
  * threaded mutual replay with per-thread accumulators, reduce).
  * accel [n][3], pot [n] in particle-id order. Returns 0 on success. */
+/* Softening guard of the MAC (extension, used by later orc_run calls with
+ * softening > 0): a pair is rejected for M2L when
+ * eps^2 (rhoA + rhoB)^3 > tau r^5; tau = 0 (default) turns it off. */
+void orc_set_soft_mac(orc_ctx*, double tau);
 int orc_run(orc_ctx*, const double* pos, const double* mass, int p, double eps_mac,
             double theta_max, double softening, int nthreads, double* accel, double* pot);
 /* Intermediates of the last orc_run (complex arrays interleaved re,im). */

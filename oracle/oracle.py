@@ -61,6 +61,7 @@ def _load():
         "orc_free": (None, [ct.c_void_p]),
         "orc_build": (ct.c_int, [ct.c_void_p, _sz, _pd, _pd, ct.c_int, ct.c_int]),
         "orc_ncells": (_sz, [ct.c_void_p]),
+        "orc_set_soft_mac": (None, [ct.c_void_p, _d]),
         "orc_max_level": (ct.c_uint32, [ct.c_void_p]),
         "orc_get_tree": (None, [ct.c_void_p, _pu32, _pu64, _pu32, _pu32, _pi32, _pu8, _pu32, _pd, _pd, _pd]),
         "orc_get_bbox": (None, [ct.c_void_p, _pd, _pd, _pd]),
@@ -370,10 +371,11 @@ class OracleFmm:
     `refresh` rebuilds, `run` evaluates and keeps every intermediate."""
 
     def __init__(self, pos, mass, p=8, eps=1e-5, n_leaf=16, center_mode=0, theta_max=1.0,
-                 softening=0.0):
+                 softening=0.0, soft_mac=0.0):
         self.p, self.eps, self.n_leaf = p, eps, n_leaf
         self.center_mode, self.theta_max, self.softening = center_mode, theta_max, softening
         self._ctx = lib().orc_new()
+        lib().orc_set_soft_mac(self._ctx, soft_mac)
         self.refresh(pos, mass)
 
     def __del__(self):

@@ -155,6 +155,33 @@ def test_forces_match_oracle(fa, case):
     assert np.abs(lg - lo).max() <= (1e-10 if p <= 12 else 1e-8) * np.abs(lo).max()
 
 
+@pytest.mark.parametrize("tau", [1e-4, 1e-5])
+def test_softening_guard_matches_oracle(fa, tau):
+    """Softening guard of the MAC (extension): lists bit-exact with the
+    oracle's guarded walk, forces to the oracle bar; the guard rejects pairs
+    (more P2P than without it) and no accepted M2L pair exceeds tau."""
+    pos, mass = O.plummer(3000, seed=4)
+    soft = 2e-2
+    say(f"[soft-guard] tau={tau}")
+    kw = dict(p=8, eps=math.inf, n_leaf=16, theta_max=0.5, softening=soft)
+    orc = O.OracleFmm(pos, mass, soft_mac=tau, **kw)
+    a_ref, pot_ref = orc.run(pos, mass)
+    gpu = fa.Fmm(pos, mass, soft_mac=tau, **kw)
+    r = gpu.run(pos, mass)
+    m2l, p2p = gpu.lists()
+    assert sorted(map(tuple, m2l)) == sorted(map(tuple, orc.m2l_pairs()))
+    assert sorted(map(tuple, p2p)) == sorted(map(tuple, orc.p2p_pairs()))
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
+    plain = fa.Fmm(pos, mass, **kw)
+    plain.run(pos, mass)
+    assert len(p2p) > len(plain.lists()[1])
+    t = gpu.tree()
+    m2l = np.asarray(m2l)
+    s = t.rho[m2l[:, 0]] + t.rho[m2l[:, 1]]
+    r2 = ((t.cen[m2l[:, 0]] - t.cen[m2l[:, 1]]) ** 2).sum(1)
+    assert np.all(soft * soft * s ** 3 <= tau * r2 ** 2.5 * (1 + 1e-12))
+
+
 def test_accel_entry_point_equals_build_plus_run(fa):
     pos, mass = O.plummer(3000, seed=21)
     say("[accel] build+evaluate in one call")

@@ -623,6 +623,40 @@ def test_softened_core_error_floor():
     assert p99[4e-3] < 1.5 * p99[0.0], p99
 
 
+def test_softening_guard():
+    """Softening guard of the MAC (extension): every accepted M2L pair keeps
+    eps^2 (rhoA + rhoB)^3 <= tau r^5, rejected pairs end in P2P or deeper
+    M2L (the forces still converge to the direct sum), tau = 0 leaves the
+    lists unchanged, and the guard only acts with softening."""
+    pos, mass = O.plummer(3000, seed=4)
+    soft = 2e-2
+    a_ref, _ = O.direct(pos, mass, softening=soft)
+    kw = dict(p=8, eps=math.inf, n_leaf=16, theta_max=0.5, softening=soft)
+    base = O.OracleFmm(pos, mass, **kw)
+    a0, _ = base.run(pos, mass)
+    lists0 = (sorted(map(tuple, base.m2l_pairs())), sorted(map(tuple, base.p2p_pairs())))
+    off = O.OracleFmm(pos, mass, soft_mac=0.0, **kw)
+    off.run(pos, mass)
+    assert (sorted(map(tuple, off.m2l_pairs())), sorted(map(tuple, off.p2p_pairs()))) == lists0
+    for tau in (1e-4, 1e-5):
+        f = O.OracleFmm(pos, mass, soft_mac=tau, **kw)
+        a, _ = f.run(pos, mass)
+        t = f.tree()
+        pr = f.m2l_pairs()
+        s = t.rho[pr[:, 0]] + t.rho[pr[:, 1]]
+        r2 = ((t.cen[pr[:, 0]] - t.cen[pr[:, 1]]) ** 2).sum(1)
+        assert np.all(soft * soft * s ** 3 <= tau * r2 ** 2.5 * (1 + 1e-12))
+        assert len(f.p2p_pairs()) > len(lists0[1])
+        rel = np.linalg.norm(a - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
+        rel0 = np.linalg.norm(a0 - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
+        assert np.percentile(rel, 99) <= 1.05 * np.percentile(rel0, 99)
+    g = O.OracleFmm(pos, mass, soft_mac=1e-5, **{**kw, "softening": 0.0})
+    g.run(pos, mass)
+    nosoft = O.OracleFmm(pos, mass, **{**kw, "softening": 0.0})
+    nosoft.run(pos, mass)
+    assert sorted(map(tuple, g.m2l_pairs())) == sorted(map(tuple, nosoft.m2l_pairs()))
+
+
 def test_softened_fmm_error_matches_unsoftened_truncation():
     """Full oracle FMM, Plummer N = 30,000, p = 8, theta = 0.5: with Plummer
     softening 1e-3 the error against the softened direct sum stays at the
