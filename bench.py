@@ -4,7 +4,9 @@
 Workload (default): Plummer sphere, N = 1,048,576 equal-mass bodies (total
 mass 1, G = 1, scale radius 1, radii truncated at 20), expansion order p = 8,
 opening angle theta = 0.5 (theta-only MAC), Plummer softening 1e-3, leaf
-capacity autotuned over 16..128, FP64. A step is one full force evaluation
+capacity autotuned over --leaf-grid (16..256), in both forms (plain FP64 and
+mixed FP32-bulk; the fastest within the error bar is reported, the other
+beside it). A step is one full force evaluation
 through the device entry point: tree build (Morton keys, radix sort, cells,
 geometry), upward pass, dual walk, interaction lists, M2L, L2L, L2P, P2P and
 the particle-id scatter, with positions and masses already resident in HBM.
@@ -39,10 +41,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-forces/sec @ rel err ≤1e-5, mixed vs FP64 GPU; p50/p99 step latency"
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
-# Softening guard of the MAC on the clustered workload (eps = 1e-4 against
-# halo cores whose leaves are a few eps across): M2L only where
+# Softening guard of the MAC on the halos and ensemble workloads (softening
+# reaching into the far field of small cells): M2L only where
 # eps^2 (rhoA + rhoB)^3 <= tau r^5.
-HALO_SOFT_MAC = 1e-4
+SOFT_MAC_DEFAULT = 1e-4
 HBM_PEAK_GBS = 8000.0
 
 
@@ -77,13 +79,13 @@ def run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, k):
 
 
 def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot, precision="fp64"):
-    """Fastest (p, n_leaf) over p in --p-grid and n_leaf in 16..128 whose
+    """Fastest (p, n_leaf) over p in --p-grid and n_leaf in --leaf-grid whose
     relative force error meets the metric's bar (99th percentile <= --max-rel-err
     against the FP64 direct sum on the sampled targets)."""
     ids = fa.sample_ids(n, args.samples, 7)
     a_ref, _, _ = fa.direct_sampled(pos, mass, ids, softening=args.softening)
     grid_p = [args.p] if args.p_grid is None else [int(x) for x in args.p_grid.split(",")]
-    leaves = [16, 24, 32, 48, 64, 96, 128]
+    leaves = [int(x) for x in args.leaf_grid.split(",")]
     best, tried = None, []
     for p in grid_p:
         for nl in leaves:
@@ -147,7 +149,7 @@ def stage_rooflines(stages_ms, wc, p):
 def bench_halos(fa, hip, args):
     n = args.n if args.n != (1 << 20) else 1 << 24
     soft = args.halo_softening
-    tau = HALO_SOFT_MAC if args.soft_mac is None else args.soft_mac
+    tau = SOFT_MAC_DEFAULT if args.soft_mac is None else args.soft_mac
     log(f"[bench] generating {n} bodies: 64 Hernquist halos in a unit cube")
     t0 = time.perf_counter()
     s = fa.ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
@@ -279,7 +281,8 @@ def bench_ensemble(fa, hip, args):
     """Batched ensemble: --systems systems, N log-uniform in [2048, 524288],
     half Plummer (r <= 20), half exponential disk (scale length 1, height
     0.05, 10 % central mass, Keplerian + self-gravity circular velocities);
-    softening 1e-3, p = 8, theta = 0.5. A step is one force evaluation of the
+    softening 1e-3 (with the softening guard of the MAC), p = 8, theta = 0.5
+    or the largest of 0.45, 0.4 within the error bar. A step is one force evaluation of the
     whole ensemble (one launch per stage); beside it, the same systems
     evaluated one after another (one context per system)."""
     rng = np.random.default_rng(7)
@@ -300,6 +303,7 @@ def bench_ensemble(fa, hip, args):
     d_pos, d_mass = hip.DeviceArray.from_host(pos), hip.DeviceArray.from_host(mass)
     d_acc, d_pot = hip.DeviceArray((n, 3)), hip.DeviceArray((n,))
     prec = args.precision or "fp32bulk"
+    tau = SOFT_MAC_DEFAULT if args.soft_mac is None else args.soft_mac
     off = np.concatenate([[0], np.cumsum(sizes)])
     # sampled targets, 16 per system, against each system's FP64 direct sum
     tgt, a_ref = [], []
@@ -313,7 +317,7 @@ def bench_ensemble(fa, hip, args):
 
     def measure(theta, n_leaf, steps, timing=False):
         ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
-                     softening=soft, timing=timing, precision=prec)
+                     softening=soft, timing=timing, precision=prec, soft_mac=tau)
         ctx.set_systems(sizes)
         run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, 1)
         el, lat, st = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, steps)
@@ -352,7 +356,7 @@ def bench_ensemble(fa, hip, args):
     seq = 0.0
     for i, s in enumerate(systems):
         one = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
-                     softening=soft, precision=prec)
+                     softening=soft, precision=prec, soft_mac=tau)
         dp, dm = hip.DeviceArray.from_host(s.pos), hip.DeviceArray.from_host(s.mass)
         da, dq = hip.DeviceArray((s.n, 3)), hip.DeviceArray((s.n,))
         one.accel_device(s.n, dp.ptr, dm.ptr, da.ptr, dq.ptr)
@@ -375,8 +379,8 @@ def bench_ensemble(fa, hip, args):
         "data": "synthetic (seeded Plummer spheres and exponential disks)",
         "config": {"workload": f"ensemble{k}_N{n}_p{args.p}_theta{theta}_soft{soft}_{prec}",
                    "systems": k, "n": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
-                   "p": args.p, "theta": theta, "softening": soft, "n_leaf": n_leaf,
-                   "precision": prec,
+                   "p": args.p, "theta": theta, "softening": soft, "soft_mac": tau,
+                   "n_leaf": n_leaf, "precision": prec,
                    "selection": "fastest (theta, n_leaf) over {0.5, 0.45, 0.4} x {32, 64} "
                                 f"with rel err p99 <= {args.max_rel_err:g}"},
         "autotune": tried,
@@ -457,7 +461,9 @@ def main():
     ap.add_argument("--p", type=int, default=8)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--softening", type=float, default=1e-3)
-    ap.add_argument("--n-leaf", type=int, default=0, help="0 = autotune over 16..128")
+    ap.add_argument("--n-leaf", type=int, default=0, help="0 = autotune over --leaf-grid")
+    ap.add_argument("--leaf-grid", default="16,24,32,48,64,96,128,192,256",
+                    help="leaf capacities the autotuner tries (with --n-leaf 0)")
     ap.add_argument("--p-grid", default="6,8,10",
                     help="expansion orders the autotuner tries (with --n-leaf 0)")
     ap.add_argument("--max-rel-err", type=float, default=1e-5,
@@ -481,8 +487,8 @@ def main():
     ap.add_argument("--halo-softening", type=float, default=1e-4,
                     help="halos: Plummer softening")
     ap.add_argument("--soft-mac", type=float, default=None,
-                    help="softening guard of the MAC (0 = off; default: halos "
-                         f"{HALO_SOFT_MAC:g}, otherwise off)")
+                    help="softening guard of the MAC (0 = off; default: halos and ensemble "
+                         f"{SOFT_MAC_DEFAULT:g}, otherwise off)")
     ap.add_argument("--eps-mac", type=float, default=math.inf,
                     help="halos: error-controlled MAC tolerance beside theta (inf = theta only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -612,7 +618,8 @@ def main():
                    "softening": args.softening, "n_leaf": n_leaf, "precision": best["precision"],
                    "form": form_names[best["precision"]],
                    "selection": (f"fastest form and (p, n_leaf) over p in {{{args.p_grid}}} x "
-                                 f"n_leaf in 16..128 with rel err p99 <= {args.max_rel_err:g}")
+                                 f"n_leaf in {{{args.leaf_grid}}} with rel err p99 <= "
+                                 f"{args.max_rel_err:g}")
                    if best["tuned"] else "fixed"},
         "error": best["err"],
         "forms": {f["precision"]: {"value": f["value"], "ms_per_step": 1e3 * f["elapsed"] / args.steps,

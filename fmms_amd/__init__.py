@@ -102,6 +102,7 @@ SIGNATURES = {
     "fmmh_stage_times": (ct.c_int, [_vp, _pd, ct.c_int]),
     "fmmh_stage_name": (ct.c_char_p, [ct.c_int]),
     "fmmh_work_counts": (ct.c_int, [_vp, _pu64]),
+    "fmmh_p2p_plan_counts": (ct.c_int, [_vp, _pu64]),
     "fmmh_direct_sampled": (ct.c_int, [_sz, _pd, _pd, ct.c_double, _sz, _pu32, _pd, _pd, _pd]),
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
@@ -365,7 +366,11 @@ class Fmm:
         self._check(lib().fmmh_work_counts(self._h, _p(out, _pu64)))
         keys = ["n", "ncells", "nleaves", "m2l_pairs", "p2p_pairs", "p2p_interactions",
                 "max_level", "walk_rounds"]
-        return dict(zip(keys, (int(v) for v in out)))
+        d = dict(zip(keys, (int(v) for v in out)))
+        pl = np.zeros(2, np.uint64)
+        self._check(lib().fmmh_p2p_plan_counts(self._h, _p(pl, _pu64)))
+        d["p2p_items"], d["p2p_split_leaves"] = int(pl[0]), int(pl[1])
+        return d
 
 
 # ------------------------------------------------------------------ models.rs

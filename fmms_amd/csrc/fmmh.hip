@@ -102,6 +102,8 @@ struct fmmh_ctx {
   Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
+  Buf pl_n, pl_items, pl_reds, pl_partial;  // P2P work plan
+  uint32_t p_nitems = 0, p_nsplit = 0;
   Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
   Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; L, G, H per cell
   int rot_p = -1;
@@ -118,7 +120,8 @@ struct fmmh_ctx {
                   &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
+                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
+                  &pl_reds, &pl_partial, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
     bufs.assign(std::begin(all), std::end(all));
@@ -439,6 +442,7 @@ struct fmmh_ctx {
     ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
     launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
                      (unsigned long long*)p2p_work.p, s);
+    plan_p2p();
     if (p <= TEMPLATED_MAX_P) {
       size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
       ensure(nch, ((size_t)nc + 1) * 4);
@@ -457,6 +461,35 @@ struct fmmh_ctx {
     }
     ensure(far, n * sizeof(double4));
     ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
+  }
+
+  // P2P work items (near.hip): per leaf the item count, partial-sum slots and
+  // split flag, scanned in place into offsets, then the items themselves.
+  void plan_p2p() {
+    p_nitems = p_nsplit = 0;
+    if (nleaves == 0) return;
+    const size_t w = (size_t)nleaves + 1;
+    ensure(pl_n, 3 * w * 4);
+    uint32_t* nitem = (uint32_t*)pl_n.p;
+    uint32_t* npart = nitem + w;
+    uint32_t* nsplit = npart + w;
+    const uint32_t* lids = (const uint32_t*)leaf_ids.p;
+    const uint32_t* ccount = (const uint32_t*)c_count.p;
+    p2p_plan_count(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount, nitem, npart,
+                   nsplit, s);
+    ensure_scan(w);
+    uint32_t* tot = nullptr;
+    scan_exclusive_u32(nitem, nitem, w, (uint32_t*)scan_tmp.p, s, &tot);
+    p_nitems = read1(tot);
+    scan_exclusive_u32(npart, npart, w, (uint32_t*)scan_tmp.p, s, &tot);
+    const uint32_t nslots = read1(tot);
+    scan_exclusive_u32(nsplit, nsplit, w, (uint32_t*)scan_tmp.p, s, &tot);
+    p_nsplit = read1(tot);
+    ensure(pl_items, (size_t)std::max<uint32_t>(p_nitems, 1) * sizeof(uint4));
+    ensure(pl_reds, (size_t)std::max<uint32_t>(p_nsplit, 1) * sizeof(uint4));
+    ensure(pl_partial, (size_t)std::max<uint32_t>(nslots, 1) * sizeof(double4));
+    p2p_plan_emit(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount, nitem, npart,
+                  nsplit, (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
   }
 
   // M2L (+ reduce), L2L, L2P and the fused P2P + scatter into acc / pot.
@@ -500,12 +533,15 @@ struct fmmh_ctx {
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
     mark(ST_P2P);
+    launch_p2p_ranges(psorted, pnnz, sbits, c, (uint2*)prange.p, s);
+    const P2PPlan pl{(const uint4*)pl_items.p, p_nitems, (const uint4*)pl_reds.p, p_nsplit,
+                     (double4*)pl_partial.p};
     if (prm.precision == FMMH_PREC_FP32BULK)
-      launch_p2p_f32((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
-                     b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
+      launch_p2p_f32(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order,
+                     acc, pot, s);
     else
-      launch_p2p((uint32_t*)leaf_ids.p, nleaves, (uint32_t*)pptr.p, psorted, pnnz, sbits, c,
-                 b.xyzm, (uint2*)prange.p, soft2, (double4*)far.p, order, acc, pot, s);
+      launch_p2p(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order, acc,
+                 pot, s);
     mark(ST_COUNT);
   }
 
@@ -1147,6 +1183,13 @@ int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8) {
   out8[5] = ctx->p2p_interactions;
   out8[6] = ctx->max_level;
   out8[7] = ctx->walk_rounds;
+  return FMMH_OK;
+}
+
+int fmmh_p2p_plan_counts(const fmmh_ctx* ctx, uint64_t* out2) {
+  if (!ctx || !out2) return FMMH_E_STATE;
+  out2[0] = ctx->p_nitems;
+  out2[1] = ctx->p_nsplit;
   return FMMH_OK;
 }
 

@@ -207,14 +207,31 @@ __device__ __forceinline__ void l2p_trace_walk(const double* __restrict__ lap,
 }
 
 // -------------------------------------------------------------- near.hip
-void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, const double4* body,
-                uint2* prange, double soft2, const double4* far, const uint32_t* order,
-                double* accel, double* pot, hipStream_t s);
-void launch_p2p_f32(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                    const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c,
-                    const double4* body, uint2* prange, double soft2, const double4* far,
-                    const uint32_t* order, double* accel, double* pot, hipStream_t s);
+// P2P work items over the leaves (see near.hip): items / reds / partial
+// live in the context; nitems >= nleaves, nsplit = leaves cut into several
+// items.
+struct P2PPlan {
+  const uint4* items;
+  uint32_t nitems;
+  const uint4* reds;
+  uint32_t nsplit;
+  double4* partial;
+};
+void p2p_plan_count(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint32_t* nitem,
+                    uint32_t* npart, uint32_t* nsplit, hipStream_t s);
+void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                   const uint64_t* skeys, int sbits, const uint32_t* ccount, const uint32_t* ioff,
+                   const uint32_t* poff, const uint32_t* roff, uint4* items, uint4* reds,
+                   hipStream_t s);
+void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
+                       hipStream_t s);
+void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
+                double soft2, const double4* far, const uint32_t* order, double* accel,
+                double* pot, hipStream_t s);
+void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
+                    double soft2, const double4* far, const uint32_t* order, double* accel,
+                    double* pot, hipStream_t s);
 // Directed body-body interactions of a P2P pair list (self pairs c(c-1));
 // out[0] = total, out needs 1 + p2p_count_scratch(np) words.
 size_t p2p_count_scratch(uint32_t np);

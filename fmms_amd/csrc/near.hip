@@ -46,6 +46,14 @@ __device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint3
 
 constexpr uint32_t P2P_RMAX = 256;  // source ranges staged in LDS per pass
 
+// P2P work items (uint4): x = target leaf | P2P_OWN when the item holds the
+// leaf's own bodies as sources, [y, z) = its slice of the leaf's neighbour
+// list, w = P2P_DIRECT (the item is the whole leaf: results go straight to
+// accel / pot) or the offset of its partial sums (one double4 per target).
+constexpr uint32_t P2P_OWN = 0x80000000u;
+constexpr uint32_t P2P_LEAF_MASK = 0x7fffffffu;
+constexpr uint32_t P2P_DIRECT = 0xffffffffu;
+
 struct TileLoad {
   double4 v;
   uint32_t fill;
@@ -76,21 +84,23 @@ __device__ __forceinline__ TileLoad gather_tile(const uint2* rng, uint32_t nr, u
   return t;
 }
 
-__global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_ids,
-                                           const uint32_t* __restrict__ pptr,
+__global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
                                            const uint2* __restrict__ prange,
                                            const uint32_t* __restrict__ cfirst,
                                            const uint32_t* __restrict__ ccount,
                                            const double4* __restrict__ body, double soft2,
                                            const double4* __restrict__ far,
                                            const uint32_t* __restrict__ order,
-                                           double* __restrict__ accel, double* __restrict__ pot) {
+                                           double* __restrict__ accel, double* __restrict__ pot,
+                                           double4* __restrict__ partial) {
   __shared__ double4 tile[64];
   __shared__ uint2 rng[P2P_RMAX];
   const uint32_t lane = threadIdx.x;
-  const uint32_t leaf = leaf_ids[xcd_block(blockIdx.x, gridDim.x)];
+  const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+  const uint32_t leaf = it.x & P2P_LEAF_MASK;
+  const bool own = (it.x & P2P_OWN) != 0u;
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
-  const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
+  const uint32_t jb = it.y, je = it.z;
   const double4 pad = make_double4(1e100, 1e100, 1e100, 0.0);
   for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
     const uint32_t nt = min(64u, nt_all - t0);
@@ -104,8 +114,9 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
     const double4 me = body[si];
     const double xi = me.x, yi = me.y, zi = me.z;
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
-    // 1) the target's own leaf, excluding each body's self term
-    for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
+    // 1) the target's own leaf (first work item of the leaf), excluding each
+    // body's self term
+    for (uint32_t s0 = 0; own && s0 < nt_all; s0 += 64) {
       __syncthreads();
       const bool in = s0 + lane < nt_all;
       double4 v = body[in ? ft + s0 + lane : ft];
@@ -145,12 +156,16 @@ __global__ void __launch_bounds__(64) k_p2p(const uint32_t* __restrict__ leaf_id
       ph += __shfl_xor(ph, (int)o, 64);
     }
     if (sub == 0 && tvalid) {
-      const double4 fv = far[si];
-      const uint32_t id = order[si];
-      accel[3 * (size_t)id] = ax + fv.x;
-      accel[3 * (size_t)id + 1] = ay + fv.y;
-      accel[3 * (size_t)id + 2] = az + fv.z;
-      if (pot) pot[id] = ph + fv.w;
+      if (it.w == P2P_DIRECT) {
+        const double4 fv = far[si];
+        const uint32_t id = order[si];
+        accel[3 * (size_t)id] = ax + fv.x;
+        accel[3 * (size_t)id + 1] = ay + fv.y;
+        accel[3 * (size_t)id + 2] = az + fv.z;
+        if (pot) pot[id] = ph + fv.w;
+      } else {
+        partial[(size_t)it.w + t0 + ti] = make_double4(ax, ay, az, ph);
+      }
     }
   }
 }
@@ -190,8 +205,7 @@ __device__ __forceinline__ float4 rel32(double4 v, double cx, double cy, double 
   return make_float4((float)(v.x - cx), (float)(v.y - cy), (float)(v.z - cz), (float)v.w);
 }
 
-__global__ void __launch_bounds__(64) k_p2p_f32(const uint32_t* __restrict__ leaf_ids,
-                                               const uint32_t* __restrict__ pptr,
+__global__ void __launch_bounds__(64) k_p2p_f32(const uint4* __restrict__ items,
                                                const uint2* __restrict__ prange,
                                                const uint32_t* __restrict__ cfirst,
                                                const uint32_t* __restrict__ ccount,
@@ -201,13 +215,16 @@ __global__ void __launch_bounds__(64) k_p2p_f32(const uint32_t* __restrict__ lea
                                                const double4* __restrict__ body, double soft2,
                                                const double4* __restrict__ far,
                                                const uint32_t* __restrict__ order,
-                                               double* __restrict__ accel, double* __restrict__ pot) {
+                                               double* __restrict__ accel, double* __restrict__ pot,
+                                               double4* __restrict__ partial) {
   __shared__ float4 tile[64];
   __shared__ uint2 rng[P2P_RMAX];
   const uint32_t lane = threadIdx.x;
-  const uint32_t leaf = leaf_ids[xcd_block(blockIdx.x, gridDim.x)];
+  const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+  const uint32_t leaf = it.x & P2P_LEAF_MASK;
+  const bool own = (it.x & P2P_OWN) != 0u;
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
-  const uint32_t jb = pptr[leaf], je = pptr[leaf + 1];
+  const uint32_t jb = it.y, je = it.z;
   const double cx = ccx[leaf], cy = ccy[leaf], cz = ccz[leaf];
   const double4 pad = make_double4(cx + 1e18, cy + 1e18, cz + 1e18, 0.0);
   const float s2f = (float)soft2;
@@ -222,7 +239,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(const uint32_t* __restrict__ lea
     const uint32_t si = ft + t0 + (tvalid ? ti : 0);
     const float4 me = rel32(body[si], cx, cy, cz);
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
-    for (uint32_t s0 = 0; s0 < nt_all; s0 += 64) {
+    for (uint32_t s0 = 0; own && s0 < nt_all; s0 += 64) {
       __syncthreads();
       const bool in = s0 + lane < nt_all;
       double4 v = body[in ? ft + s0 + lane : ft];
@@ -268,12 +285,16 @@ __global__ void __launch_bounds__(64) k_p2p_f32(const uint32_t* __restrict__ lea
       ph += __shfl_xor(ph, (int)o, 64);
     }
     if (sub == 0 && tvalid) {
-      const double4 fv = far[si];
-      const uint32_t id = order[si];
-      accel[3 * (size_t)id] = ax + fv.x;
-      accel[3 * (size_t)id + 1] = ay + fv.y;
-      accel[3 * (size_t)id + 2] = az + fv.z;
-      if (pot) pot[id] = ph + fv.w;
+      if (it.w == P2P_DIRECT) {
+        const double4 fv = far[si];
+        const uint32_t id = order[si];
+        accel[3 * (size_t)id] = ax + fv.x;
+        accel[3 * (size_t)id + 1] = ay + fv.y;
+        accel[3 * (size_t)id + 2] = az + fv.z;
+        if (pot) pot[id] = ph + fv.w;
+      } else {
+        partial[(size_t)it.w + t0 + ti] = make_double4(ax, ay, az, ph);
+      }
     }
   }
 }
@@ -290,30 +311,226 @@ __global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__
   prange[j] = make_uint2(cfirst[src], ccount[src]);
 }
 
-void launch_p2p_f32(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                    const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c,
-                    const double4* body, uint2* prange, double soft2, const double4* far,
-                    const uint32_t* order, double* accel, double* pot, hipStream_t s) {
-  if (nleaves == 0) return;
-  const uint64_t smask = (1ull << sbits) - 1ull;
-  if (nnz)
-    hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
-                       c.first, c.count, prange);
-  hipLaunchKernelGGL(k_p2p_f32, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, prange, c.first,
-                     c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot);
+// ------------------------------------------------------- P2P work plan
+// A leaf's P2P work is its target count times its source bodies (own leaf
+// plus neighbour leaves). One block per leaf leaves the whole device waiting
+// on the few leaves whose neighbourhoods are huge (a large sparse leaf beside
+// a dense region), so a leaf whose work exceeds P2P_ITEM_WORK is cut into
+// m = ceil(work / P2P_ITEM_WORK) work items (at most one per source unit).
+// Source units are the own leaf (unit 0) and the neighbour ranges in list
+// order; unit u with P_u sources before it goes to item
+// min(m - 1, P_u / ceil(S / m)) (items may come out empty). Split items write
+// partial sums that k_p2p_reduce adds in item order. The cut depends on the
+// leaf's own counts only, so a system evaluates the same alone and in an
+// ensemble. One wave per leaf in both plan kernels.
+constexpr uint64_t P2P_ITEM_WORK = 1ull << 20;
+constexpr int PLAN_T = 256;
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-void launch_p2p(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, const double4* body,
-                uint2* prange, double soft2, const double4* far, const uint32_t* order,
-                double* accel, double* pot, hipStream_t s) {
+struct LeafCut {
+  uint32_t leaf, nt, jb, je;
+  uint64_t S, m;
+};
+
+// Source total and item count of leaf i (all lanes get the result).
+__device__ __forceinline__ LeafCut leaf_cut(uint32_t i, const uint32_t* __restrict__ leaf_ids,
+                                            const uint32_t* __restrict__ pptr,
+                                            const uint64_t* __restrict__ skeys, uint64_t smask,
+                                            const uint32_t* __restrict__ ccount, uint32_t lane) {
+  LeafCut L;
+  L.leaf = leaf_ids[i];
+  L.nt = ccount[L.leaf];
+  L.jb = pptr[L.leaf];
+  L.je = pptr[L.leaf + 1];
+  uint64_t s = 0;
+  for (uint32_t j = L.jb + lane; j < L.je; j += 64) {
+    const uint32_t src = (uint32_t)(skeys[j] & smask);
+    s += src == L.leaf ? 0u : ccount[src];
+  }
+  L.S = (uint64_t)L.nt + wave_sum_u64(s);
+  const uint64_t work = (uint64_t)L.nt * L.S;
+  uint64_t m = work > P2P_ITEM_WORK ? (work + P2P_ITEM_WORK - 1) / P2P_ITEM_WORK : 1;
+  const uint64_t units = (uint64_t)(L.je - L.jb) + 1;
+  L.m = m > units ? units : m;
+  return L;
+}
+
+__global__ void __launch_bounds__(PLAN_T) k_p2p_plan_count(const uint32_t* __restrict__ leaf_ids,
+                                                          uint32_t nleaves,
+                                                          const uint32_t* __restrict__ pptr,
+                                                          const uint64_t* __restrict__ skeys,
+                                                          uint64_t smask,
+                                                          const uint32_t* __restrict__ ccount,
+                                                          uint32_t* __restrict__ nitem,
+                                                          uint32_t* __restrict__ npart,
+                                                          uint32_t* __restrict__ nsplit) {
+  const uint32_t i = blockIdx.x * (PLAN_T / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (i > nleaves) return;
+  if (i == nleaves) {  // the scans' closing element
+    if (lane == 0) nitem[i] = npart[i] = nsplit[i] = 0u;
+    return;
+  }
+  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, lane);
+  if (lane == 0) {
+    const uint32_t m = (uint32_t)L.m;
+    nitem[i] = m;
+    npart[i] = m > 1 ? m * L.nt : 0u;
+    nsplit[i] = m > 1 ? 1u : 0u;
+  }
+}
+
+__global__ void __launch_bounds__(PLAN_T) k_p2p_plan_emit(const uint32_t* __restrict__ leaf_ids,
+                                                         uint32_t nleaves,
+                                                         const uint32_t* __restrict__ pptr,
+                                                         const uint64_t* __restrict__ skeys,
+                                                         uint64_t smask,
+                                                         const uint32_t* __restrict__ ccount,
+                                                         const uint32_t* __restrict__ ioff,
+                                                         const uint32_t* __restrict__ poff,
+                                                         const uint32_t* __restrict__ roff,
+                                                         uint4* __restrict__ items,
+                                                         uint4* __restrict__ reds) {
+  const uint32_t i = blockIdx.x * (PLAN_T / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (i >= nleaves) return;
+  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, lane);
+  const uint32_t m = (uint32_t)L.m;
+  const uint32_t base = ioff[i], pb = poff[i];
+  uint32_t* it = (uint32_t*)(items + base);  // [m][4]: leaf|own, begin, end, slot
+  for (uint32_t k = lane; k < m; k += 64) {
+    it[4 * k] = L.leaf | (k == 0 ? P2P_OWN : 0u);
+    it[4 * k + 3] = m > 1 ? pb + k * L.nt : P2P_DIRECT;
+  }
+  if (lane == 0) {
+    it[1] = L.jb;
+    if (m > 1) reds[roff[i]] = make_uint4(L.leaf, pb, m, 0u);
+  }
+  // item boundaries: where the unit's item index steps up
+  const uint64_t T = (L.S + L.m - 1) / L.m;
+  uint64_t running = L.nt;  // sources before the first neighbour unit
+  uint32_t gprev = 0;       // item of unit 0 (the own leaf)
+  for (uint32_t c0 = L.jb; c0 < L.je; c0 += 64) {
+    const uint32_t j = c0 + lane;
+    const bool valid = j < L.je;
+    uint64_t cnt = 0;
+    if (valid) {
+      const uint32_t src = (uint32_t)(skeys[j] & smask);
+      cnt = src == L.leaf ? 0u : ccount[src];
+    }
+    uint64_t incl = cnt;  // inclusive wave scan
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t v = __shfl_up(incl, o, 64);
+      if ((int)lane >= o) incl += v;
+    }
+    const uint64_t P = running + incl - cnt;
+    const uint64_t gq = P / T;
+    const uint32_t g = valid ? (uint32_t)(gq < L.m - 1 ? gq : L.m - 1) : 0u;
+    const uint32_t gl = (uint32_t)__shfl_up(g, 1, 64);
+    const uint32_t before = lane == 0 ? gprev : gl;
+    if (valid && g > before)
+      for (uint32_t k = before + 1; k <= g; k++) {
+        it[4 * k + 1] = j;
+        it[4 * (k - 1) + 2] = j;
+      }
+    const uint32_t last = min(63u, L.je - 1 - c0);  // last valid lane of this chunk
+    gprev = (uint32_t)__shfl(g, (int)last, 64);
+    running += __shfl(incl, (int)last, 64);
+  }
+  // items after the last unit's item are empty; the last item ends the list
+  for (uint32_t k = gprev + lane; k < m; k += 64) {
+    if (k > gprev) it[4 * k + 1] = L.je;
+    it[4 * k + 2] = L.je;
+  }
+}
+
+// Split leaves: far field + the items' partial sums in item order, scattered
+// to particle-id order.
+__global__ void __launch_bounds__(64) k_p2p_reduce(const uint4* __restrict__ reds,
+                                                  const uint32_t* __restrict__ cfirst,
+                                                  const uint32_t* __restrict__ ccount,
+                                                  const double4* __restrict__ partial,
+                                                  const double4* __restrict__ far,
+                                                  const uint32_t* __restrict__ order,
+                                                  double* __restrict__ accel,
+                                                  double* __restrict__ pot) {
+  const uint4 r = reds[blockIdx.x];
+  const uint32_t ft = cfirst[r.x], nt = ccount[r.x];
+  for (uint32_t t = threadIdx.x; t < nt; t += 64) {
+    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    for (uint32_t k = 0; k < r.z; k++) {
+      const double4 v = partial[(size_t)r.y + (size_t)k * nt + t];
+      ax += v.x;
+      ay += v.y;
+      az += v.z;
+      ph += v.w;
+    }
+    const double4 fv = far[ft + t];
+    const uint32_t id = order[ft + t];
+    accel[3 * (size_t)id] = ax + fv.x;
+    accel[3 * (size_t)id + 1] = ay + fv.y;
+    accel[3 * (size_t)id + 2] = az + fv.z;
+    if (pot) pot[id] = ph + fv.w;
+  }
+}
+
+// Work plan of the current P2P lists. nitem / npart / nsplit are nleaves + 1
+// words each (scanned in place); the totals come back through the host.
+void p2p_plan_count(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint32_t* nitem,
+                    uint32_t* npart, uint32_t* nsplit, hipStream_t s) {
   if (nleaves == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
-  if (nnz)
-    hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
-                       c.first, c.count, prange);
-  hipLaunchKernelGGL(k_p2p, dim3(nleaves), dim3(64), 0, s, leaf_ids, pptr, prange, c.first,
-                     c.count, body, soft2, far, order, accel, pot);
+  hipLaunchKernelGGL(k_p2p_plan_count, dim3((nleaves + 1 + PLAN_T / 64 - 1) / (PLAN_T / 64)),
+                     dim3(PLAN_T), 0, s, leaf_ids,
+                     nleaves, pptr, skeys, smask, ccount, nitem, npart, nsplit);
+}
+
+void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
+                   const uint64_t* skeys, int sbits, const uint32_t* ccount, const uint32_t* ioff,
+                   const uint32_t* poff, const uint32_t* roff, uint4* items, uint4* reds,
+                   hipStream_t s) {
+  if (nleaves == 0) return;
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  hipLaunchKernelGGL(k_p2p_plan_emit, dim3((nleaves + PLAN_T / 64 - 1) / (PLAN_T / 64)),
+                     dim3(PLAN_T), 0, s, leaf_ids,
+                     nleaves, pptr, skeys, smask, ccount, ioff, poff, roff, items, reds);
+}
+
+void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
+                       hipStream_t s) {
+  if (nnz == 0) return;
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
+                     c.first, c.count, prange);
+}
+
+void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
+                    double soft2, const double4* far, const uint32_t* order, double* accel,
+                    double* pot, hipStream_t s) {
+  if (pl.nitems == 0) return;
+  hipLaunchKernelGGL(k_p2p_f32, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
+                     c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
+  if (pl.nsplit)
+    hipLaunchKernelGGL(k_p2p_reduce, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
+                       pl.partial, far, order, accel, pot);
+}
+
+void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
+                double soft2, const double4* far, const uint32_t* order, double* accel,
+                double* pot, hipStream_t s) {
+  if (pl.nitems == 0) return;
+  hipLaunchKernelGGL(k_p2p, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first, c.count,
+                     body, soft2, far, order, accel, pot, pl.partial);
+  if (pl.nsplit)
+    hipLaunchKernelGGL(k_p2p_reduce, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
+                       pl.partial, far, order, accel, pot);
 }
 
 __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,

@@ -191,6 +191,11 @@ const char* fmmh_stage_name(int stage);
 /* Work counts of the last evaluation: [n, ncells, nleaves, n_m2l_mutual,
  * n_p2p_pairs(incl. self), p2p_directed_interactions, max_level, walk_rounds]. */
 int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8);
+/* P2P work plan of the current lists (no counterpart in the reference, whose
+ * threads take leaf pairs from a shared list): [work items, leaves cut into
+ * several items]. A leaf whose targets x sources exceed 2^20 is cut into
+ * items of about equal source count, added back in a fixed order. */
+int fmmh_p2p_plan_counts(const fmmh_ctx* ctx, uint64_t* out2);
 
 /* FP64 direct sum at sampled targets, on the device: acc [nt][3], pot [nt]
  * and f [nt] (= sum m/r^2) for target ids idx[nt]; host arrays. */

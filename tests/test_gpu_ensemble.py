@@ -93,6 +93,27 @@ def test_ensemble_leapfrog_equals_standalone(fa):
         assert np.array_equal(bits(v[lo:hi]), bits(v1)), f"system {i}"
 
 
+def test_ensemble_with_split_p2p_leaves_equals_standalone(fa):
+    """Systems whose large leaves are cut into several P2P work items: the cut
+    depends on each leaf's own counts only, so every system still matches
+    its standalone evaluation bit for bit."""
+    systems = [fa.ic_plummer(20000, rmax=20.0, seed=71), fa.ic_hernquist(15000, a=0.5, rmax=30.0,
+                                                                         seed=72)]
+    say("[ensemble] split P2P leaves, n_leaf=256")
+    pos = np.concatenate([s.pos for s in systems])
+    mass = np.concatenate([s.mass for s in systems])
+    kw = dict(p=6, eps=math.inf, n_leaf=256, theta_max=0.5, softening=1e-3)
+    g = fa.Fmm(None, None, **kw)
+    off = g.set_systems([s.n for s in systems])
+    r = g.accel(pos, mass)
+    assert g.work_counts()["p2p_split_leaves"] > 0
+    for i, s in enumerate(systems):
+        ri = fa.Fmm(None, None, **kw).accel(s.pos, s.mass)
+        a, b = int(off[i]), int(off[i + 1])
+        assert np.array_equal(bits(r.accel[a:b]), bits(ri.accel)), f"system {i}"
+        assert np.array_equal(bits(r.pot[a:b]), bits(ri.pot)), f"system {i}"
+
+
 def test_ensemble_argument_checks(fa):
     g = fa.Fmm(None, None, p=8, eps=1e-5, n_leaf=16)
     with pytest.raises(fa.FmmError):  # error-controlled MAC is single-system only

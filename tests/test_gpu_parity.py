@@ -182,6 +182,34 @@ def test_softening_guard_matches_oracle(fa, tau):
     assert np.all(soft * soft * s ** 3 <= tau * r2 ** 2.5 * (1 + 1e-12))
 
 
+def test_p2p_split_leaves_match_oracle(fa):
+    """Large leaves (n_leaf = 256) beside dense regions carry P2P work above
+    the per-item bound and are cut into several work items whose partial
+    sums are added in item order: lists bit-exact, forces to the oracle bar
+    in FP64; the mixed form agrees with FP64 to FP32 pair precision; both are
+    deterministic run to run."""
+    pos, mass = O.plummer(20000, seed=8)
+    kw = dict(p=6, eps=math.inf, n_leaf=256, theta_max=0.5, softening=1e-3)
+    say("[p2p-split] N=20000 n_leaf=256")
+    orc = O.OracleFmm(pos, mass, **kw)
+    a_ref, pot_ref = orc.run(pos, mass)
+    gpu = fa.Fmm(pos, mass, **kw)
+    r = gpu.run(pos, mass)
+    wc = gpu.work_counts()
+    assert wc["p2p_split_leaves"] > 0 and wc["p2p_items"] > wc["nleaves"], wc
+    m2l, p2p = gpu.lists()
+    assert sorted(map(tuple, p2p)) == sorted(map(tuple, orc.p2p_pairs()))
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
+    r2 = gpu.run(pos, mass)
+    assert np.array_equal(r.accel, r2.accel) and np.array_equal(r.pot, r2.pot)
+    g32 = fa.Fmm(pos, mass, precision="fp32bulk", **kw)
+    q1 = g32.run(pos, mass)
+    q2 = g32.run(pos, mass)
+    assert np.array_equal(q1.accel, q2.accel) and np.array_equal(q1.pot, q2.pot)
+    rel = np.linalg.norm(q1.accel - r.accel, axis=1) / np.linalg.norm(r.accel, axis=1)
+    assert np.percentile(rel, 99) < 1e-6 and rel.max() < 5e-5
+
+
 def test_accel_entry_point_equals_build_plus_run(fa):
     pos, mass = O.plummer(3000, seed=21)
     say("[accel] build+evaluate in one call")
