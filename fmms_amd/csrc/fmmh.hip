@@ -103,6 +103,7 @@ struct fmmh_ctx {
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
   Buf pl_n, pl_items, pl_reds, pl_partial;  // P2P work plan
+  Buf m2l_lam, m2l_m32;                     // mixed form: FP32 M2L scales, scaled moments
   uint32_t p_nitems = 0, p_nsplit = 0;
   Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
   Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; L, G, H per cell
@@ -121,7 +122,7 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
-                  &pl_reds, &pl_partial, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
+                  &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
     bufs.assign(std::begin(all), std::end(all));
@@ -456,6 +457,10 @@ struct fmmh_ctx {
       HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
       nchunks = read1(tot);
       ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
+      if (prm.precision == FMMH_PREC_FP32BULK) {
+        ensure(m2l_lam, (size_t)nc * sizeof(double));
+        ensure(m2l_m32, (size_t)nc * m2l32_moment_bytes(p));
+      }
     } else if (p >= 20) {
       rot_tables();
     }
@@ -504,7 +509,11 @@ struct fmmh_ctx {
     const bool gen = p > TEMPLATED_MAX_P;
     mark(ST_M2L);
     if (!gen) {
-      launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, soft2, (cplx*)part.p, s);
+      if (prm.precision == FMMH_PREC_FP32BULK)
+        launch_m2l32(p, nc, chunks.p, nchunks, msorted, sbits, c, (const BBox*)bbox.p, keys_sorted,
+                     vlev, mom, soft2, (double*)m2l_lam.p, m2l_m32.p, (cplx*)part.p, s);
+      else
+        launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, soft2, (cplx*)part.p, s);
       mark(ST_M2L_REDUCE);
       launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
     } else {

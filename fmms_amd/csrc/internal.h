@@ -136,6 +136,13 @@ void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t
 bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                        int sbits, DevCells c, const cplx* moments, double soft2, cplx* part,
                        hipStream_t s);
+// Mixed form: the same M2L on radius-scaled FP32 operands (m2l_kernel.h);
+// lam [nc] doubles and m32 [nc] x m2l32_moment_bytes(p) are scratch.
+bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
+                  int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
+                  const cplx* moments, double soft2, double* lam, void* m32, cplx* part,
+                  hipStream_t s);
+size_t m2l32_moment_bytes(int p);
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s);
 

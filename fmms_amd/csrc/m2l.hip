@@ -9,6 +9,52 @@ namespace fmmh {
 template <int P>
 void m2l_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
            DevCells c, const cplx* moments, double soft2, cplx* part, hipStream_t s);
+template <int P>
+void m2l32_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
+             DevCells c, const double* lam, const cplxf* m32, double soft2, cplx* part,
+             hipStream_t s);
+
+// Scale of each cell for the FP32 M2L: its radius, floored at a quarter of
+// its octant edge (the system's box side halved per level below the system
+// root) so cells of coincident bodies still get a length; 1 if both vanish.
+__global__ void __launch_bounds__(256) k_m2l_scale(uint32_t nc, DevCells c,
+                                                  const BBox* __restrict__ bb,
+                                                  const uint64_t* __restrict__ keys_sorted,
+                                                  int vlev, double* __restrict__ lam) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  const int lev = (int)c.level[i];
+  double l = 1.0;
+  if (lev >= vlev) {
+    const uint32_t sys = vlev ? (uint32_t)(keys_sorted[c.first[i]] >> (63 - 3 * vlev)) : 0u;
+    l = fmax(c.rho[i], 0.25 * ldexp(bb[sys].side, -(lev - vlev)));
+    if (!(l > 0.0) || !isfinite(l)) l = 1.0;
+  }
+  lam[i] = l;
+}
+
+// Moments of every cell scaled by lam^-k, m >= 0 half, FP32.
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_prep32(uint32_t nc, const cplx* __restrict__ moments,
+                                                   const double* __restrict__ lam,
+                                                   cplxf* __restrict__ m32) {
+  constexpr int D = hdim(P), NF = hnpos(P);
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nc) return;
+  const cplx* ms = moments + (size_t)i * D;
+  cplxf* o = m32 + (size_t)i * NF;
+  const double il = 1.0 / lam[i];
+  double sc = 1.0;
+#pragma unroll
+  for (int k = 0; k <= P; k++) {
+#pragma unroll
+    for (int l = 0; l <= k; l++) {
+      const cplx v = ms[hidx(k, l)];
+      o[hpos(k, l)] = cplxf{(float)(v.re * sc), (float)(v.im * sc)};
+    }
+    sc *= il;
+  }
+}
 
 __global__ void __launch_bounds__(256) k_m2l_nchunks(const uint32_t* __restrict__ tptr, uint32_t nc,
                                                     uint32_t* __restrict__ nch) {
@@ -102,6 +148,26 @@ bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64
 #undef CALL
   return true;
 }
+
+bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
+                  int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
+                  const cplx* moments, double soft2, double* lam, void* m32, cplx* part,
+                  hipStream_t s) {
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  const M2lChunk* ck = (const M2lChunk*)chunks;
+  if (nc == 0) return true;
+  hipLaunchKernelGGL(k_m2l_scale, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, bb, keys_sorted,
+                     vlev, lam);
+#define CALL(P)                                                                            \
+  hipLaunchKernelGGL(k_m2l_prep32<P>, dim3((nc + 255) / 256), dim3(256), 0, s, nc, moments, \
+                     (const double*)lam, (cplxf*)m32);                                     \
+  m2l32_t<P>(ck, nchunks, skeys, smask, c, lam, (const cplxf*)m32, soft2, part, s)
+  FMMH_P_SWITCH3(p, CALL)
+#undef CALL
+  return true;
+}
+
+size_t m2l32_moment_bytes(int p) { return (size_t)hnpos(p) * sizeof(cplxf); }
 
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s) {
