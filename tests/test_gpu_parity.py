@@ -182,6 +182,46 @@ def test_softening_guard_matches_oracle(fa, tau):
     assert np.all(soft * soft * s ** 3 <= tau * r2 ** 2.5 * (1 + 1e-12))
 
 
+@pytest.mark.parametrize("p", [4, 8, 10])
+def test_fp32_m2l_locals_match_fp64(fa, p):
+    """The mixed form's FP32 M2L (per-target scale lam = max(rho, octant edge
+    / 4), moments pre-scaled by lam_s^-k, dimensionless offsets) against the
+    FP64 M2L: after L2L every cell's local expansion within 5e-6 of the FP64
+    one in the cell's scaled norm sum_n |F_n| lam^n, for 99 % of cells, and
+    within 2e-5 for all (FP32 sums of up to 16 sources; a scaling slip would
+    show as O(1) errors or overflow)."""
+    pos, mass = O.plummer(8000, seed=81)
+    say(f"[fp32-m2l] p={p}")
+    kw = dict(p=p, eps=math.inf, n_leaf=16, theta_max=0.5, softening=1e-3)
+    g64 = fa.Fmm(pos, mass, **kw)
+    g64.run(pos, mass)
+    g32 = fa.Fmm(pos, mass, precision="fp32bulk", **kw)
+    g32.run(pos, mass)
+    l64, l32 = g64.locals(), g32.locals()
+    t = g64.tree()
+    n_of = np.concatenate([[n] * (2 * n + 1) for n in range(p + 1)])
+    w = np.maximum(t.rho, 1e-12)[:, None] ** n_of[None, :]  # lam^n with lam ~ rho
+    num = (np.abs(l32 - l64) * w).sum(1)
+    den = (np.abs(l64) * w).sum(1)
+    live = den > 0
+    rel = num[live] / den[live]
+    assert np.percentile(rel, 99) < 5e-6 and rel.max() < 2e-5, (np.percentile(rel, 99), rel.max())
+
+
+def test_fp32bulk_coincident_bodies(fa):
+    """Cells of coincident bodies (radius 0) still get an FP32 M2L scale (a
+    quarter of their octant edge): the mixed form stays finite and within
+    FP32 precision of the FP64 path."""
+    pos, mass = dup_cloud()
+    say("[fp32bulk] coincident bodies")
+    kw = dict(p=8, eps=math.inf, n_leaf=4, theta_max=0.5, softening=1e-3)
+    r64 = fa.Fmm(None, None, **kw).accel(pos, mass)
+    r32 = fa.Fmm(None, None, precision="fp32bulk", **kw).accel(pos, mass)
+    assert np.all(np.isfinite(r32.accel)) and np.all(np.isfinite(r32.pot))
+    d = np.linalg.norm(r32.accel - r64.accel, axis=1) / np.linalg.norm(r64.accel, axis=1)
+    assert np.percentile(d, 99) < 1e-5 and d.max() < 1e-4, (np.percentile(d, 99), d.max())
+
+
 def test_p2p_split_leaves_match_oracle(fa):
     """Large leaves (n_leaf = 256) beside dense regions carry P2P work above
     the per-item bound and are cut into several work items whose partial
@@ -339,8 +379,9 @@ def test_large_theta_mac_properties(fa):
 
 @pytest.mark.parametrize("soft,n_leaf", [(1e-3, 32), (0.0, 96), (1e-2, 64)])
 def test_fp32bulk_form_accuracy(fa, soft, n_leaf):
-    """FP32-bulk form (P2P pair terms in FP32 on leaf-centred positions, FP64
-    tile and final sums): 99 % of bodies within 1e-6 of the FP64 path, every
+    """FP32-bulk form (P2P pair terms in FP32 on leaf-centred positions, M2L
+    in FP32 on radius-scaled operands, FP64 tile, chunk and final sums): 99 %
+    of bodies within 1e-6 of the FP64 path, every
     body within 5e-5 (an unsoftened close pair in a large leaf keeps only the
     FP32 resolution of leaf-centred coordinates), and no worse than the FP64
     path against the FP64 direct sum (the metric's bar)."""
