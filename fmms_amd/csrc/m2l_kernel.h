@@ -20,7 +20,7 @@
 namespace fmmh {
 
 #ifndef FMMH_M2L_CH
-#define FMMH_M2L_CH 16
+#define FMMH_M2L_CH 32
 #endif
 constexpr int M2L_CH = FMMH_M2L_CH;  // sources per chunk (one thread's serial work)
 

@@ -323,7 +323,10 @@ __global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__
 // partial sums that k_p2p_reduce adds in item order. The cut depends on the
 // leaf's own counts only, so a system evaluates the same alone and in an
 // ensemble. One wave per leaf in both plan kernels.
-constexpr uint64_t P2P_ITEM_WORK = 1ull << 20;
+#ifndef FMMH_P2P_ITEM_WORK
+#define FMMH_P2P_ITEM_WORK (1ull << 18)
+#endif
+constexpr uint64_t P2P_ITEM_WORK = FMMH_P2P_ITEM_WORK;
 constexpr int PLAN_T = 256;
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {

@@ -193,7 +193,7 @@ const char* fmmh_stage_name(int stage);
 int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8);
 /* P2P work plan of the current lists (no counterpart in the reference, whose
  * threads take leaf pairs from a shared list): [work items, leaves cut into
- * several items]. A leaf whose targets x sources exceed 2^20 is cut into
+ * several items]. A leaf whose targets x sources exceed 2^18 is cut into
  * items of about equal source count, added back in a fixed order. */
 int fmmh_p2p_plan_counts(const fmmh_ctx* ctx, uint64_t* out2);
 

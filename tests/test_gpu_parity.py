@@ -188,7 +188,7 @@ def test_fp32_m2l_locals_match_fp64(fa, p):
     / 4), moments pre-scaled by lam_s^-k, dimensionless offsets) against the
     FP64 M2L: after L2L every cell's local expansion within 5e-6 of the FP64
     one in the cell's scaled norm sum_n |F_n| lam^n, for 99 % of cells, and
-    within 2e-5 for all (FP32 sums of up to 16 sources; a scaling slip would
+    within 2e-5 for all (FP32 sums of up to 32 sources; a scaling slip would
     show as O(1) errors or overflow)."""
     pos, mass = O.plummer(8000, seed=81)
     say(f"[fp32-m2l] p={p}")
