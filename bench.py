@@ -171,6 +171,28 @@ def roofline_line(stage_roof, dom, extra=None):
     return out
 
 
+def body_weighted_targets(fa, systems, k, softening, seed=7):
+    """k targets drawn uniformly over all bodies of a set of systems (each
+    body-force counts once, as in the metric) and their FP64 direct-sum
+    accelerations, each against its own system. Returns (global ids, a_ref)."""
+    sizes = np.array([s.n for s in systems])
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    ids = np.sort(fa.sample_ids(int(off[-1]), k, seed).astype(np.int64))
+    a_ref = np.empty((len(ids), 3))
+    sys_of = np.searchsorted(off, ids, side="right") - 1
+    for i in np.unique(sys_of):
+        sel = np.nonzero(sys_of == i)[0]
+        s = systems[i]
+        a_ref[sel] = fa.direct_sampled(s.pos, s.mass, (ids[sel] - off[i]).astype(np.uint32),
+                                       softening=softening)[0]
+    return ids, a_ref
+
+
+def error_stats(rel):
+    return {"targets": int(rel.size), "rel_median": float(np.median(rel)),
+            "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max())}
+
+
 def bench_halos(fa, hip, args):
     n = args.n if args.n != (1 << 20) else 1 << 24
     soft = args.halo_softening
@@ -329,14 +351,17 @@ def bench_ensemble(fa, hip, args):
     tau = SOFT_MAC_DEFAULT if args.soft_mac is None else args.soft_mac
     off = np.concatenate([[0], np.cumsum(sizes)])
     # sampled targets, 16 per system, against each system's FP64 direct sum
-    tgt, a_ref = [], []
+    # Error: 4096 targets drawn uniformly over all bodies (the bar), and beside
+    # it 16 per system (every system weighs the same; small systems dominate).
+    tgt, a_ref, tgt_s, a_ref_s = [], [], [], []
     if not args.no_error:
-        log("[bench] FP64 direct sums on 16 sampled targets per system")
+        log("[bench] FP64 direct sums: 4096 body-weighted targets, 16 per system")
+        tgt, a_ref = body_weighted_targets(fa, systems, args.samples, soft)
         for i, s in enumerate(systems):
             ids = fa.sample_ids(s.n, 16, 7 + i)
-            a_ref.append(fa.direct_sampled(s.pos, s.mass, ids, softening=soft)[0])
-            tgt.append(off[i] + ids.astype(np.int64))
-        tgt, a_ref = np.concatenate(tgt), np.concatenate(a_ref)
+            a_ref_s.append(fa.direct_sampled(s.pos, s.mass, ids, softening=soft)[0])
+            tgt_s.append(off[i] + ids.astype(np.int64))
+        tgt_s, a_ref_s = np.concatenate(tgt_s), np.concatenate(a_ref_s)
 
     def measure(theta, n_leaf, steps, timing=False):
         ctx = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
@@ -348,6 +373,7 @@ def bench_ensemble(fa, hip, args):
         ctx.close()
         acc = d_acc.download()
         rel = fa.rel_errors(acc[tgt], a_ref) if len(tgt) else None
+        measure.per_system = fa.rel_errors(acc[tgt_s], a_ref_s) if len(tgt_s) else None
         return el, lat, st, wc, rel
 
     # fastest (theta, n_leaf) meeting the error bar (p fixed by the workload)
@@ -388,9 +414,10 @@ def bench_ensemble(fa, hip, args):
             x.free()
     err = None
     if rel is not None:
-        err = {"targets": int(rel.size), "rel_median": float(np.median(rel)),
-               "rel_p99": float(np.percentile(rel, 99)), "rel_max": float(rel.max())}
-        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
+        err = error_stats(rel)
+        err["per_system_16"] = error_stats(measure.per_system)
+        log(f"[bench] rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e} "
+            f"(16 per system: p99 {err['per_system_16']['rel_p99']:.3e})")
     out = {
         "metric": METRIC, "value": n * args.steps / elapsed, "unit": "body-forces/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -404,7 +431,8 @@ def bench_ensemble(fa, hip, args):
                    "p": args.p, "theta": theta, "softening": soft, "soft_mac": tau,
                    "n_leaf": n_leaf, "precision": prec,
                    "selection": "fastest (theta, n_leaf) over {0.5, 0.45, 0.4} x {32, 64} "
-                                f"with rel err p99 <= {args.max_rel_err:g}"},
+                                f"with rel err p99 <= {args.max_rel_err:g} over 4096 "
+                                "targets drawn uniformly over all bodies"},
         "autotune": tried,
         "one_at_a_time": {"value": n / seq, "ms_per_pass": 1e3 * seq,
                           "speedup_batched": (n * args.steps / elapsed) / (n / seq)},
@@ -435,8 +463,40 @@ def bench_serve(fa, hip, args):
         f"{total_steps} body-steps, arrivals over {t:.1f} s")
     n_leaf = args.n_leaf or 32
     prec = args.precision or "fp32bulk"
-    srv = serve.Server(p=args.p, theta=args.theta, softening=1e-3, n_leaf=n_leaf, dt=1e-4,
-                       hbm_budget=200e9, precision=prec)
+    tau = SOFT_MAC_DEFAULT if args.soft_mac is None else args.soft_mac
+    # Force error of the served evaluations: each system in the batch gets
+    # bit for bit its standalone forces, so every request's initial system is
+    # evaluated alone with the server's parameters and checked against its FP64
+    # direct sum on 16 sampled targets; the largest theta meeting the bar serves.
+    # 4096 targets drawn uniformly over all served bodies (the bar) and 16 per
+    # request beside it (every system weighs the same).
+    tried, theta, err = [], args.theta, None
+    if not args.no_error:
+        bw_ids, bw_ref = body_weighted_targets(fa, reqs, args.samples, 1e-3)
+        off = np.concatenate([[0], np.cumsum([r.n for r in reqs])])
+        ps_ids = [fa.sample_ids(r.n, 16, 7 + i) for i, r in enumerate(reqs)]
+        ps_ref = [fa.direct_sampled(r.pos, r.mass, ids, softening=1e-3)[0]
+                  for r, ids in zip(reqs, ps_ids)]
+    for th in ([] if args.no_error else [0.5, 0.45, 0.4]):
+        acc = []
+        for r in reqs:
+            f = fa.Fmm(None, None, p=args.p, eps=math.inf, n_leaf=n_leaf, theta_max=th,
+                       softening=1e-3, precision=prec, soft_mac=tau)
+            acc.append(f.accel(r.pos, r.mass).accel)
+            f.close()
+        acc = np.concatenate(acc)
+        e = {"theta": th, **error_stats(fa.rel_errors(acc[bw_ids], bw_ref))}
+        e["per_system_16"] = error_stats(np.concatenate(
+            [fa.rel_errors(acc[off[i] + ids.astype(np.int64)], a)
+             for i, (ids, a) in enumerate(zip(ps_ids, ps_ref))]))
+        tried.append(e)
+        log(f"[bench] serve theta={th}: rel err median {e['rel_median']:.2e} p99 {e['rel_p99']:.2e} "
+            f"(16 per request: p99 {e['per_system_16']['rel_p99']:.2e})")
+        theta, err = th, e
+        if e["rel_p99"] <= args.max_rel_err:
+            break
+    srv = serve.Server(p=args.p, theta=theta, softening=1e-3, n_leaf=n_leaf, dt=1e-4,
+                       hbm_budget=200e9, precision=prec, soft_mac=tau)
     wall = srv.run(reqs)
     st = srv.stats()
     lat = np.array([r.finished - r.arrival for r in reqs])
@@ -446,8 +506,8 @@ def bench_serve(fa, hip, args):
     alone = 0.0
     alone_ms = []
     for r in reqs:
-        _, el = serve.run_alone(r, p=args.p, theta=args.theta, softening=1e-3, n_leaf=n_leaf,
-                                dt=1e-4)
+        _, el = serve.run_alone(r, p=args.p, theta=theta, softening=1e-3, n_leaf=n_leaf,
+                                dt=1e-4, soft_mac=tau)
         alone += el
         alone_ms.append(1e3 * el / r.steps)
     out = {
@@ -458,10 +518,15 @@ def bench_serve(fa, hip, args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == "fp64" else "f64+f32",
         "data": "synthetic (seeded Plummer, Hernquist and disk systems, Poisson arrivals)",
-        "config": {"workload": f"serve_{len(reqs)}req_rate{args.rate}_p{args.p}_theta{args.theta}",
+        "config": {"workload": f"serve_{len(reqs)}req_rate{args.rate}_p{args.p}_theta{theta}",
                    "requests": len(reqs), "rate_per_s": args.rate, "p": args.p,
-                   "theta": args.theta, "softening": 1e-3, "n_leaf": n_leaf, "dt": 1e-4,
-                   "hbm_budget_bytes": 200e9, "precision": prec},
+                   "theta": theta, "softening": 1e-3, "soft_mac": tau, "n_leaf": n_leaf,
+                   "dt": 1e-4, "hbm_budget_bytes": 200e9, "precision": prec,
+                   "selection": "largest theta in {0.5, 0.45, 0.4} with rel err p99 <= "
+                                f"{args.max_rel_err:g} over 4096 targets drawn uniformly over "
+                                "all requests' initial bodies"},
+        "error": err,
+        "theta_trials": tried,
         "serving": {"wall_s": wall, "body_steps": total_steps,
                     "wall_body_forces_per_s": total_steps / wall,
                     "request_latency_p50_s": float(np.percentile(lat, 50)),

@@ -78,9 +78,9 @@ class Server:
     FIELDS = (("pos", 3), ("vel", 3), ("mass", 1), ("acc", 3), ("pot", 1))
 
     def __init__(self, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4,
-                 hbm_budget=200e9, precision="fp64"):
+                 hbm_budget=200e9, precision="fp64", soft_mac=0.0):
         self.kw = dict(p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening,
-                       precision=precision)
+                       precision=precision, soft_mac=soft_mac)
         self.dt = dt
         self.hbm_budget = float(hbm_budget)
         self.ctx = Fmm(None, None, **self.kw)
@@ -231,11 +231,12 @@ class Server:
                 "mean_batch_bodies": float(bodies.mean())}
 
 
-def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4):
+def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4, soft_mac=0.0):
     """The same system stepped on its own (the reference behaviour: one Sim)."""
     from . import Integrator
 
-    f = Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening)
+    f = Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening,
+            soft_mac=soft_mac)
     it = Integrator(f, req.pos, req.vel, req.mass)
     it.step(dt, steps=0)
     t0 = time.perf_counter()

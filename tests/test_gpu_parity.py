@@ -250,6 +250,31 @@ def test_p2p_split_leaves_match_oracle(fa):
     assert np.percentile(rel, 99) < 1e-6 and rel.max() < 5e-5
 
 
+def test_lists_with_long_segments(fa):
+    """A cell with thousands of P2P neighbours (a large sparse leaf beside a
+    dense clump): lists and forces match the oracle, reruns are
+    bit-identical."""
+    rng = np.random.default_rng(5)
+    clump = rng.random((24000, 3)) * 0.02 + np.array([0.5, 0.5, 0.5])
+    pos = np.vstack([clump, rng.random((400, 3))])
+    mass = np.full(len(pos), 1.0 / len(pos))
+    kw = dict(p=4, eps=math.inf, n_leaf=8, theta_max=0.5, softening=1e-3)
+    say("[lists] long segments")
+    orc = O.OracleFmm(pos, mass, **kw)
+    a_ref, pot_ref = orc.run(pos, mass)
+    deg = np.bincount(orc.p2p_pairs().ravel(), minlength=orc.ncells)
+    assert deg.max() > 2048
+    gpu = fa.Fmm(pos, mass, **kw)
+    r = gpu.run(pos, mass)
+    m2l, p2p = gpu.lists()
+    assert sorted(map(tuple, m2l)) == sorted(map(tuple, orc.m2l_pairs()))
+    assert sorted(map(tuple, p2p)) == sorted(map(tuple, orc.p2p_pairs()))
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
+    for _ in range(2):
+        r2 = fa.Fmm(pos, mass, **kw).run(pos, mass)
+        assert np.array_equal(r.accel, r2.accel) and np.array_equal(r.pot, r2.pot)
+
+
 def test_accel_entry_point_equals_build_plus_run(fa):
     pos, mass = O.plummer(3000, seed=21)
     say("[accel] build+evaluate in one call")
