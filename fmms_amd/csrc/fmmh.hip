@@ -408,7 +408,7 @@ struct fmmh_ctx {
     ensure(pows, (size_t)nc * (p + 1) * 8);
     ensure(fvals, n * 8);
     launch_pows(p, nc, (const cplx*)moments.p, (double*)fact.p, (double*)pows.p, s);
-    launch_fprepass((uint32_t)n, nc, c, b, (double*)fvals.p, s);
+    launch_fprepass((uint32_t)n, nc, c, b, keys_sorted, vlev, (double*)fvals.p, s);
     for (int L = (int)max_level; L >= 0; L--)
       launch_fmin((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c,
                   (double*)fvals.p, (double*)fmin.p, s);
@@ -906,8 +906,6 @@ int fmmh_set_systems(fmmh_ctx* ctx, size_t nsys, const uint64_t* offsets) {
   for (size_t i = 0; i < nsys; i++)
     if (offsets[i + 1] < offsets[i]) return fail(ctx, FMMH_E_PARAM, "offsets must not decrease");
   if (offsets[nsys] >= 0xFFFFFFFFull) return fail(ctx, FMMH_E_PARAM, "n must be < 2^32");
-  if (nsys > 1 && std::isfinite(ctx->prm.eps_mac))
-    return fail(ctx, FMMH_E_PARAM, "ensembles use the theta MAC (eps_mac = INFINITY)");
   ctx->nsys = (uint32_t)nsys;
   ctx->vlev = 0;
   while ((1ull << (3 * ctx->vlev)) < nsys) ctx->vlev++;

@@ -91,7 +91,8 @@ bool launch_m2m(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* mo
                 hipStream_t s);
 void launch_pows(int p, uint32_t nc, const cplx* moments, const double* fact, double* pows,
                  hipStream_t s);
-void launch_fprepass(uint32_t n, uint32_t nc, DevCells c, DevBodies b, double* f, hipStream_t s);
+void launch_fprepass(uint32_t n, uint32_t nc, DevCells c, DevBodies b,
+                     const uint64_t* keys_sorted, int vlev, double* f, hipStream_t s);
 void launch_fmin(const uint32_t* cells, uint32_t cnt, DevCells c, const double* f, double* fmin,
                  hipStream_t s);
 

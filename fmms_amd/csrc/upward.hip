@@ -225,6 +225,10 @@ void launch_pows(int p, uint32_t nc, const cplx* moments, const double* fact, do
 // lane that accepts a cell's monopole sleeps until the wave reaches that
 // cell's rope, so each lane accumulates precisely its own reference
 // sequence of terms, in order. Cell and body loads are wave-uniform.
+// Ensembles (vlev > 0): a lane walks only its own system's subtree. A
+// virtual cell (level < vlev) is opened by the lanes whose system it holds
+// and skipped by the others; a real cell of another system is skipped; so
+// every lane meets its system's cells in the standalone order.
 struct FpCells {
   const uint8_t* __restrict__ leaf;
   const uint32_t* __restrict__ first;
@@ -236,6 +240,7 @@ struct FpCells {
   const double* __restrict__ mass;
   const int32_t* __restrict__ rope;
   const int32_t* __restrict__ last_child;
+  const uint8_t* __restrict__ level;
 };
 
 __global__ void __launch_bounds__(64) k_fprepass(uint32_t n, uint32_t max_steps, FpCells c,
@@ -243,10 +248,13 @@ __global__ void __launch_bounds__(64) k_fprepass(uint32_t n, uint32_t max_steps,
                                                 const double* __restrict__ by,
                                                 const double* __restrict__ bz,
                                                 const double* __restrict__ bm,
-                                                double* __restrict__ f) {
+                                                const uint64_t* __restrict__ keys_sorted,
+                                                int vlev, double* __restrict__ f) {
   const int ACTIVE = -2, NEVER = -3;
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   const bool valid = i < n;
+  const int sh = 63 - 3 * vlev;
+  const uint32_t mysys = (valid && vlev) ? (uint32_t)(keys_sorted[i] >> sh) : 0u;
   const double px = valid ? bx[i] : 0.0, py = valid ? by[i] : 0.0, pz = valid ? bz[i] : 0.0;
   double fi = 0.0;
   int resume = valid ? ACTIVE : NEVER;
@@ -256,7 +264,21 @@ __global__ void __launch_bounds__(64) k_fprepass(uint32_t n, uint32_t max_steps,
   for (uint32_t step = 0; cur >= 0 && step < max_steps; step++) {
     cur = wave_uniform(cur);
     if (resume == cur) resume = ACTIVE;
-    const bool act = resume == ACTIVE;
+    bool act = resume == ACTIVE;
+    bool virt = false;
+    if (vlev) {
+      const uint32_t f0 = c.first[cur];
+      const uint32_t slo = (uint32_t)(keys_sorted[f0] >> sh);
+      virt = (int)c.level[cur] < vlev;
+      if (act) {
+        const uint32_t shi =
+            virt ? (uint32_t)(keys_sorted[f0 + c.count[cur] - 1] >> sh) : slo;
+        if (mysys < slo || mysys > shi) {  // not this lane's system
+          resume = c.rope[cur];
+          act = false;
+        }
+      }
+    }
     if (c.leaf[cur]) {
       const uint32_t f0 = c.first[cur], e = f0 + c.count[cur];
       for (uint32_t k = f0; k < e; k++) {
@@ -273,7 +295,9 @@ __global__ void __launch_bounds__(64) k_fprepass(uint32_t n, uint32_t max_steps,
       const double rh = c.rho[cur], ms = c.mass[cur];
       const int rope = c.rope[cur];
       bool open = false;
-      if (act) {
+      if (act && virt) {
+        open = true;  // a virtual cell on the way to the lane's system root
+      } else if (act) {
         double dx = px - cxv, dy = py - cyv, dz = pz - czv;
         double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > rh * rh) {
@@ -289,10 +313,12 @@ __global__ void __launch_bounds__(64) k_fprepass(uint32_t n, uint32_t max_steps,
   if (valid) f[i] = fi;
 }
 
-void launch_fprepass(uint32_t n, uint32_t nc, DevCells c, DevBodies b, double* f, hipStream_t s) {
-  FpCells fc{c.leaf, c.first, c.count, c.cx, c.cy, c.cz, c.rho, c.mass, c.rope, c.last_child};
+void launch_fprepass(uint32_t n, uint32_t nc, DevCells c, DevBodies b,
+                     const uint64_t* keys_sorted, int vlev, double* f, hipStream_t s) {
+  FpCells fc{c.leaf, c.first, c.count, c.cx, c.cy, c.cz, c.rho, c.mass, c.rope, c.last_child,
+             c.level};
   hipLaunchKernelGGL(k_fprepass, dim3((n + 63) / 64), dim3(64), 0, s, n, nc + 1, fc, b.x, b.y, b.z,
-                     b.m, f);
+                     b.m, keys_sorted, vlev, f);
 }
 
 // fmin (mac.rs:151-173), one level at a time, deepest first.

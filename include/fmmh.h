@@ -112,8 +112,9 @@ size_t fmmh_num_cells(const fmmh_ctx* ctx);
  * at 21 - V bits per axis), walked from the systems' roots so no interaction
  * crosses systems; every stage runs once for the whole ensemble. Every
  * evaluation and leapfrog entry point then works on the ensemble (the
- * leapfrog's dt clamp uses the ensemble-wide maximum acceleration). Theta
- * MAC only (eps_mac = INFINITY). nsys = 1 restores the single-system mode. */
+ * leapfrog's dt clamp uses the ensemble-wide maximum acceleration). Both
+ * MACs work per system (the error-controlled MAC's f pre-pass walks each
+ * body's own system only). nsys = 1 restores the single-system mode. */
 #define FMMH_MAX_SYSTEMS 4096
 int fmmh_set_systems(fmmh_ctx* ctx, size_t nsys, const uint64_t* offsets);
 

@@ -114,10 +114,32 @@ def test_ensemble_with_split_p2p_leaves_equals_standalone(fa):
         assert np.array_equal(bits(r.pot[a:b]), bits(ri.pot)), f"system {i}"
 
 
+@pytest.mark.parametrize("eps,theta", [(1e-6, 1.0), (1e-5, 0.6)])
+def test_ensemble_error_mac_equals_standalone(fa, eps, theta):
+    """Error-controlled MAC (mac.rs:23-95, the reference's default) on an
+    ensemble: the f pre-pass walks each body's own system only, so f, fmin,
+    the lists and the forces of every system equal its standalone run bit for
+    bit (systems overlap in space, so a cross-system term would show)."""
+    systems = make_systems(fa, 13, seed=29)
+    say(f"[ensemble] error MAC eps={eps} theta={theta}")
+    pos = np.concatenate([s.pos for s in systems])
+    mass = np.concatenate([s.mass for s in systems])
+    kw = dict(p=8, eps=eps, n_leaf=32, theta_max=theta, softening=1e-3)
+    g = fa.Fmm(None, None, **kw)
+    off = g.set_systems([s.n for s in systems])
+    r = g.accel(pos, mass)
+    f_all, _ = g.f()
+    for i, s in enumerate(systems):
+        one = fa.Fmm(None, None, **kw)
+        ri = one.accel(s.pos, s.mass)
+        fi, _ = one.f()
+        a, b = int(off[i]), int(off[i + 1])
+        assert np.array_equal(bits(r.accel[a:b]), bits(ri.accel)), f"system {i}"
+        assert np.array_equal(bits(r.pot[a:b]), bits(ri.pot)), f"system {i}"
+        assert np.array_equal(np.sort(f_all[a:b]), np.sort(fi)), f"system {i}"
+
+
 def test_ensemble_argument_checks(fa):
-    g = fa.Fmm(None, None, p=8, eps=1e-5, n_leaf=16)
-    with pytest.raises(fa.FmmError):  # error-controlled MAC is single-system only
-        g.set_systems([100, 200])
     g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=16, theta_max=0.5)
     g.set_systems([100, 200])
     pos = np.random.default_rng(1).random((250, 3))
