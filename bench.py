@@ -461,7 +461,7 @@ def bench_serve(fa, hip, args):
     total_steps = sum(r.n * r.steps for r in reqs)
     log(f"[bench] {len(reqs)} requests, {sum(r.n for r in reqs)} bodies, "
         f"{total_steps} body-steps, arrivals over {t:.1f} s")
-    n_leaf = args.n_leaf or 32
+    n_leaf = args.n_leaf or 64
     prec = args.precision or "fp32bulk"
     tau = SOFT_MAC_DEFAULT if args.soft_mac is None else args.soft_mac
     # Force error of the served evaluations: each system in the batch gets
