@@ -508,12 +508,13 @@ struct fmmh_ctx {
     const double soft2 = prm.softening * prm.softening;
     const bool gen = p > TEMPLATED_MAX_P;
     mark(ST_M2L);
-    if (!gen) {
-      if (prm.precision == FMMH_PREC_FP32BULK)
-        launch_m2l32(p, nc, chunks.p, nchunks, msorted, sbits, c, (const BBox*)bbox.p, keys_sorted,
-                     vlev, mom, soft2, (double*)m2l_lam.p, m2l_m32.p, (cplx*)part.p, s);
-      else
-        launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, soft2, (cplx*)part.p, s);
+    if (!gen && prm.precision == FMMH_PREC_FP32BULK) {
+      launch_m2l32(p, nc, chunks.p, nchunks, msorted, sbits, c, (const BBox*)bbox.p, keys_sorted,
+                   vlev, mom, soft2, (double*)m2l_lam.p, m2l_m32.p, part.p, s);
+      mark(ST_M2L_REDUCE);
+      launch_m2l_reduce32(p, nc, (uint32_t*)choff.p, part.p, (const double*)m2l_lam.p, loc, s);
+    } else if (!gen) {
+      launch_m2l_kernel(p, chunks.p, nchunks, msorted, sbits, c, mom, soft2, (cplx*)part.p, s);
       mark(ST_M2L_REDUCE);
       launch_m2l_reduce(p, nc, (uint32_t*)choff.p, (cplx*)part.p, loc, s);
     } else {

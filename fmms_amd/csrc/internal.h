@@ -141,9 +141,11 @@ bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64
 // lam [nc] doubles and m32 [nc] x m2l32_moment_bytes(p) are scratch.
 bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                   int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
-                  const cplx* moments, double soft2, double* lam, void* m32, cplx* part,
+                  const cplx* moments, double soft2, double* lam, void* m32, void* part,
                   hipStream_t s);
 size_t m2l32_moment_bytes(int p);
+bool launch_m2l_reduce32(int p, uint32_t nc, const uint32_t* choff, const void* part,
+                         const double* lam, cplx* locals, hipStream_t s);
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s);
 

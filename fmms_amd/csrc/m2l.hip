@@ -11,7 +11,7 @@ void m2l_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint
            DevCells c, const cplx* moments, double soft2, cplx* part, hipStream_t s);
 template <int P>
 void m2l32_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
-             DevCells c, const double* lam, const cplxf* m32, double soft2, cplx* part,
+             DevCells c, const double* lam, const cplxf* m32, double soft2, cplxf* part,
              hipStream_t s);
 
 // Scale of each cell for the FP32 M2L: its radius, floored at a quarter of
@@ -102,6 +102,39 @@ __global__ void __launch_bounds__(256) k_m2l_reduce(uint32_t nc, const uint32_t*
   }
 }
 
+// Mixed form: FP32 chunk partials of the scaled F, summed in FP64 in chunk
+// order, then F_n = lam_t^-(n+1) * sum.
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_reduce32(uint32_t nc, const uint32_t* __restrict__ choff,
+                                                     const cplxf* __restrict__ part,
+                                                     const double* __restrict__ lam,
+                                                     cplx* __restrict__ locals) {
+  constexpr int D = hdim(P), NF = hnpos(P);
+  size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= (size_t)nc * NF) return;
+  uint32_t t = (uint32_t)(g / NF);
+  int e = (int)(g % NF);
+  int n = 0;
+  while ((n + 1) * (n + 2) / 2 <= e) n++;
+  int m = e - n * (n + 1) / 2;
+  double sr = 0.0, si = 0.0;
+  for (uint32_t c = choff[t]; c < choff[t + 1]; c++) {
+    const cplxf v = part[(size_t)c * NF + e];
+    sr += (double)v.re;
+    si += (double)v.im;
+  }
+  const double il = 1.0 / lam[t];
+  double sc = il;
+  for (int k = 0; k < n; k++) sc *= il;
+  const cplx s = cplx{sr * sc, si * sc};
+  cplx* out = locals + (size_t)t * D;
+  out[hidx(n, m)] = s;
+  if (m > 0) {
+    double sg = (m & 1) ? -1.0 : 1.0;
+    out[hidx(n, -m)] = cplx{sg * s.re, -sg * s.im};
+  }
+}
+
 // Scratch layout handed in by the context: nch/choff (nc+1 words each) and
 // chunk/partial arrays sized for the worst case sum_t ceil(S_t/CH).
 size_t m2l_max_chunks(uint32_t nnz, uint32_t nc) { return (size_t)nnz / M2L_CH + nc + 1; }
@@ -151,7 +184,7 @@ bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64
 
 bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                   int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
-                  const cplx* moments, double soft2, double* lam, void* m32, cplx* part,
+                  const cplx* moments, double soft2, double* lam, void* m32, void* part,
                   hipStream_t s) {
   const uint64_t smask = (1ull << sbits) - 1ull;
   const M2lChunk* ck = (const M2lChunk*)chunks;
@@ -161,13 +194,23 @@ bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, cons
 #define CALL(P)                                                                            \
   hipLaunchKernelGGL(k_m2l_prep32<P>, dim3((nc + 255) / 256), dim3(256), 0, s, nc, moments, \
                      (const double*)lam, (cplxf*)m32);                                     \
-  m2l32_t<P>(ck, nchunks, skeys, smask, c, lam, (const cplxf*)m32, soft2, part, s)
+  m2l32_t<P>(ck, nchunks, skeys, smask, c, lam, (const cplxf*)m32, soft2, (cplxf*)part, s)
   FMMH_P_SWITCH3(p, CALL)
 #undef CALL
   return true;
 }
 
 size_t m2l32_moment_bytes(int p) { return (size_t)hnpos(p) * sizeof(cplxf); }
+
+bool launch_m2l_reduce32(int p, uint32_t nc, const uint32_t* choff, const void* part,
+                         const double* lam, cplx* locals, hipStream_t s) {
+#define CALL(P)                                                                                  \
+  hipLaunchKernelGGL(k_m2l_reduce32<P>, dim3(((size_t)nc * hnpos(P) + 255) / 256), dim3(256), 0, \
+                     s, nc, choff, (const cplxf*)part, lam, locals)
+  FMMH_P_SWITCH3(p, CALL)
+#undef CALL
+  return true;
+}
 
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s) {

@@ -225,9 +225,9 @@ void m2l_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint
 //   F_n = lam_t^-(n+1) sum_k (M_k lam_t^-k) Theta_{n+k}(v).
 // Moments are stored pre-scaled per source cell, Mt_k = M_k lam_s^-k (FP32),
 // and M_k lam_t^-k = Mt_k kappa^k with kappa = lam_s / lam_t. Every operand
-// is then O(1) up to the MAC's (rho / r)^k decay, the recursion, products and
-// the chunk's sums run in FP32, and the chunk's F is rescaled to FP64 on the
-// way out (the reduce over chunks stays FP64).
+// is then O(1) up to the MAC's (rho / r)^k decay; the recursion, products and
+// the chunk's sums run in FP32. Chunks write their scaled F in FP32; the
+// reduce adds a target's chunks in FP64 and applies lam_t^-(n+1).
 template <int P, int H>
 __device__ __forceinline__ void m2l_chunk32(const M2lChunk ck, uint32_t ci,
                                             const uint64_t* __restrict__ skeys, uint64_t smask,
@@ -236,7 +236,7 @@ __device__ __forceinline__ void m2l_chunk32(const M2lChunk ck, uint32_t ci,
                                             const double* __restrict__ ccz,
                                             const double* __restrict__ lam,
                                             const cplxf* __restrict__ m32, double soft2,
-                                            cplx* __restrict__ part) {
+                                            cplxf* __restrict__ part) {
   constexpr int NF = hnpos(P);
   const double tx = ccx[ck.target], ty = ccy[ck.target], tz = ccz[ck.target];
   const double ilt = 1.0 / lam[ck.target];
@@ -263,16 +263,12 @@ __device__ __forceinline__ void m2l_chunk32(const M2lChunk ck, uint32_t ci,
                                        (float)((ty - ccy[src]) * ilt),
                                        (float)((tz - ccz[src]) * ilt), s2s, M, F);
   }
-  cplx* out = part + (size_t)ci * NF;
-  double sc = ilt;  // lam_t^-(n+1)
+  cplxf* out = part + (size_t)ci * NF;
 #pragma unroll
   for (int n = 0; n <= P; n++) {
-    if (H == 2 || (n & 1) == H) {
+    if (H != 2 && (n & 1) != H) continue;
 #pragma unroll
-      for (int m = 0; m <= n; m++)
-        out[hpos(n, m)] = cplx{(double)F[hpos(n, m)].re * sc, (double)F[hpos(n, m)].im * sc};
-    }
-    sc *= ilt;
+    for (int m = 0; m <= n; m++) out[hpos(n, m)] = F[hpos(n, m)];
   }
 }
 
@@ -284,7 +280,7 @@ __global__ void __launch_bounds__(64) k_m2l32(const M2lChunk* __restrict__ chunk
                                              const double* __restrict__ ccz,
                                              const double* __restrict__ lam,
                                              const cplxf* __restrict__ m32, double soft2,
-                                             cplx* __restrict__ part) {
+                                             cplxf* __restrict__ part) {
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   if constexpr (SPLIT) {
     const uint32_t ci = (lb >> 1) * blockDim.x + threadIdx.x;
@@ -307,7 +303,7 @@ __global__ void __launch_bounds__(64) k_m2l32(const M2lChunk* __restrict__ chunk
 
 template <int P>
 void m2l32_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, uint64_t smask,
-             DevCells c, const double* lam, const cplxf* m32, double soft2, cplx* part,
+             DevCells c, const double* lam, const cplxf* m32, double soft2, cplxf* part,
              hipStream_t s) {
   if (!nchunks) return;
   constexpr bool split = P >= FMMH_M2L32_SPLIT_FROM;
@@ -321,6 +317,6 @@ void m2l32_t(const M2lChunk* chunks, uint32_t nchunks, const uint64_t* skeys, ui
   template void m2l_t<P>(const M2lChunk*, uint32_t, const uint64_t*, uint64_t, DevCells,    \
                          const cplx*, double, cplx*, hipStream_t);                          \
   template void m2l32_t<P>(const M2lChunk*, uint32_t, const uint64_t*, uint64_t, DevCells,  \
-                           const double*, const cplxf*, double, cplx*, hipStream_t);
+                           const double*, const cplxf*, double, cplxf*, hipStream_t);
 
 }  // namespace fmmh
