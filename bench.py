@@ -25,7 +25,8 @@ kick-drift-kick leapfrog step at dt = 1e-4 on device-resident state. The
 tree and lists are rebuilt every --refresh steps; the steps in between
 replay one captured hipGraph holding the whole step (dt, kicks, drift,
 upward pass, M2L, L2L, L2P, P2P). Both forms (--forms) run the same
-trajectory; the fastest one within the error bar is reported, FP64 beside it.
+trajectory at each leaf capacity of --halo-leaf-grid; the fastest form and
+capacity within the error bar is reported, FP64 at its best beside it.
 """
 import argparse
 import json
@@ -209,7 +210,7 @@ def bench_halos(fa, hip, args):
     d_vel = hip.DeviceArray((n, 3))
     d_acc = hip.DeviceArray((n, 3))
     d_pot = hip.DeviceArray((n,))
-    n_leaf = args.n_leaf or 64
+    leaf_grid = [args.n_leaf] if args.n_leaf else [int(x) for x in args.halo_leaf_grid.split(",")]
     ids = fa.sample_ids(n, args.samples, 7)
     nh = int(math.floor(0.9 * n + 0.5))  # halo bodies come first
 
@@ -225,17 +226,19 @@ def bench_halos(fa, hip, args):
         hip.synchronize()
         return time.perf_counter() - t
 
-    def measure(precision):
-        """Ten KDK steps of one precision form (whole step in a captured
-        graph), per-stage times of one full and one frozen evaluation, and
-        the error of the final forces against the FP64 direct sum."""
+    def measure(precision, n_leaf):
+        """Ten KDK steps of one precision form and leaf capacity (whole step
+        in a captured graph), per-stage times of one full and one frozen
+        evaluation, and the error of the final forces against the FP64 direct
+        sum."""
         mk = lambda timing=False: fa.Fmm(None, None, p=args.p, eps=args.eps_mac, n_leaf=n_leaf,
                                           theta_max=args.theta, softening=soft, timing=timing,
                                           precision=precision, soft_mac=tau)
         ctx = mk()
-        log(f"[bench] {precision}: warm-up trajectory")
+        log(f"[bench] {precision} n_leaf={n_leaf}: warm-up trajectory")
         trajectory(ctx, max(1, args.warmup))
-        log(f"[bench] {precision}: timing {args.steps} KDK steps (refresh every {args.refresh})")
+        log(f"[bench] {precision} n_leaf={n_leaf}: timing {args.steps} KDK steps "
+            f"(refresh every {args.refresh})")
         st0 = ctx.leapfrog_stats()
         elapsed = trajectory(ctx, args.steps)
         st1 = ctx.leapfrog_stats()
@@ -274,10 +277,20 @@ def bench_halos(fa, hip, args):
             err = {"targets": int(len(ids)), **stats(acc), "fresh_tree": stats(acc_fresh)}
             log(f"[bench] {precision}: rel err median {err['rel_median']:.3e} p99 "
                 f"{err['rel_p99']:.3e}; fresh tree p99 {err['fresh_tree']['rel_p99']:.3e}")
-        return {"precision": precision, "elapsed": elapsed, "lf": lf, "wc": wc, "err": err,
-                "full_ms": full_ms, "frozen_ms": frozen_ms, "value": n * args.steps / elapsed}
+        return {"precision": precision, "n_leaf": n_leaf, "elapsed": elapsed, "lf": lf, "wc": wc,
+                "err": err, "full_ms": full_ms, "frozen_ms": frozen_ms,
+                "value": n * args.steps / elapsed}
 
-    forms = [measure(f) for f in args.forms.split(",")]
+    def ok(r):
+        return r["err"] is None or r["err"]["rel_p99"] <= args.max_rel_err
+
+    forms, tuned = [], {}
+    for prec in args.forms.split(","):
+        runs = [measure(prec, nl) for nl in leaf_grid]
+        tuned[prec] = [{"n_leaf": r["n_leaf"], "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                        "rel_p99": r["err"]["rel_p99"] if r["err"] else None, "valid": ok(r)}
+                       for r in runs]
+        forms.append(max([r for r in runs if ok(r)] or runs, key=lambda r: r["value"]))
     valid = [f for f in forms if f["err"] is None or f["err"]["rel_p99"] <= args.max_rel_err]
     best = max(valid or forms, key=lambda f: f["value"])
     fp64 = next((f for f in forms if f["precision"] == "fp64"), None)
@@ -301,14 +314,16 @@ def bench_halos(fa, hip, args):
                    "n": n, "p": args.p, "theta": args.theta, "softening": soft,
                    "soft_mac": tau,
                    "eps_mac": "inf" if math.isinf(args.eps_mac) else args.eps_mac,
-                   "n_leaf": n_leaf, "dt": args.dt, "refresh_every": args.refresh,
+                   "n_leaf": best["n_leaf"], "dt": args.dt, "refresh_every": args.refresh,
                    "drift_tol": 0.25, "graph": True, "precision": best["precision"],
-                   "selection": f"fastest form with rel err p99 <= {args.max_rel_err:g}"},
+                   "selection": f"fastest form and n_leaf in {{{','.join(map(str, leaf_grid))}}} "
+                                f"with rel err p99 <= {args.max_rel_err:g}"},
+        "autotune": tuned,
         "valid": bool(valid),
         "leapfrog": best["lf"],
         "error": best["err"],
         "forms": {f["precision"]: {"value": f["value"], "ms_per_step": 1e3 * f["elapsed"] / args.steps,
-                                   "error": f["err"], "leapfrog": f["lf"],
+                                   "n_leaf": f["n_leaf"], "error": f["err"], "leapfrog": f["lf"],
                                    "stages_ms_frozen": {k: round(v, 4) for k, v in f["frozen_ms"].items()}}
                   for f in forms},
         "mixed_over_fp64": (best["value"] / fp64["value"]) if fp64 else None,
@@ -572,6 +587,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=10,
                     help="halos: rebuild the tree and lists every this many steps")
     ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--halo-leaf-grid", default="64,96,128,192",
+                    help="halos: leaf capacities tried per form (with --n-leaf 0)")
     ap.add_argument("--halo-softening", type=float, default=1e-4,
                     help="halos: Plummer softening")
     ap.add_argument("--soft-mac", type=float, default=None,
