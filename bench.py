@@ -394,7 +394,7 @@ def bench_ensemble(fa, hip, args):
     # fastest (theta, n_leaf) meeting the error bar (p fixed by the workload)
     best, tried = None, []
     for theta in ([args.theta] if args.n_leaf else [0.5, 0.45, 0.4]):
-        for nl in ([args.n_leaf] if args.n_leaf else [32, 64]):
+        for nl in ([args.n_leaf] if args.n_leaf else [32, 64, 96, 128]):
             el, _, _, _, rel = measure(theta, nl, 2)
             p99 = float(np.percentile(rel, 99)) if rel is not None else 0.0
             ok = p99 <= args.max_rel_err
@@ -445,7 +445,7 @@ def bench_ensemble(fa, hip, args):
                    "systems": k, "n": n, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
                    "p": args.p, "theta": theta, "softening": soft, "soft_mac": tau,
                    "n_leaf": n_leaf, "precision": prec,
-                   "selection": "fastest (theta, n_leaf) over {0.5, 0.45, 0.4} x {32, 64} "
+                   "selection": "fastest (theta, n_leaf) over {0.5, 0.45, 0.4} x {32, 64, 96, 128} "
                                 f"with rel err p99 <= {args.max_rel_err:g} over 4096 "
                                 "targets drawn uniformly over all bodies"},
         "autotune": tried,
