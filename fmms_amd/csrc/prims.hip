@@ -136,6 +136,12 @@ __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint64_t* __restrict__ k
     hist[(size_t)i * nblk + blockIdx.x] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
 }
 
+// Scatter of one 8-bit pass. Each wave ranks its keys per digit (ballot
+// peer masks, in element order); the block's keys are then reordered by digit
+// in LDS (stable: wave order, then element order within a wave) and written
+// from LDS in that order, so the global writes of one digit are contiguous
+// runs instead of one line per lane.
+template <bool VALS>
 __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict__ kin,
                                                     const uint32_t* __restrict__ vin,
                                                     uint64_t* __restrict__ kout,
@@ -144,10 +150,14 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict_
                                                     uint32_t nblk) {
   __shared__ uint32_t wcnt[4][256];
   __shared__ uint32_t wpre[4][256];
+  __shared__ uint32_t dstart[256];
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ uint32_t sv[VALS ? RS_TILE : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 4 * 256; i += RS_T) (&wcnt[0][0])[i] = 0;
   __syncthreads();
-  const size_t wbase = (size_t)blockIdx.x * RS_TILE + (size_t)w * RS_WAVE_SPAN;
+  const size_t bbase = (size_t)blockIdx.x * RS_TILE;
+  const size_t wbase = bbase + (size_t)w * RS_WAVE_SPAN;
   const uint64_t lt = lanemask_lt();
   uint64_t kk[RS_I];
   uint32_t vv[RS_I];
@@ -158,7 +168,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict_
     size_t e = wbase + (size_t)j * 64 + lane;
     bool valid = e < n;
     uint64_t key = valid ? kin[e] : 0ull;
-    uint32_t val = (valid && vin) ? vin[e] : 0u;
+    uint32_t val = (VALS && valid) ? vin[e] : 0u;
     uint32_t d = (uint32_t)(key >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -179,22 +189,43 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict_
     dg[j] = valid ? d : 0xFFFFFFFFu;
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < 256; d += RS_T) {
-    uint32_t s = 0;
+  // per digit: offsets of the waves inside the digit's block run, and the
+  // block's digit total; then the digits' starts in the block (exclusive scan)
+  uint32_t tot = 0;
+  {
+    const int d = threadIdx.x;  // RS_T == 256 digits
 #pragma unroll
     for (int ww = 0; ww < 4; ww++) {
       uint32_t c = wcnt[ww][d];
-      wpre[ww][d] = s;
-      s += c;
+      wpre[ww][d] = tot;
+      tot += c;
     }
+    dstart[d] = tot;
   }
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan of the digit totals
+    const uint32_t add = threadIdx.x >= o ? dstart[threadIdx.x - o] : 0u;
+    __syncthreads();
+    dstart[threadIdx.x] += add;
+    __syncthreads();
+  }
+  dstart[threadIdx.x] -= tot;  // exclusive
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < RS_I; j++) {
     if (dg[j] == 0xFFFFFFFFu) continue;
-    size_t pos = (size_t)offs[(size_t)dg[j] * nblk + blockIdx.x] + wpre[w][dg[j]] + lp[j];
-    kout[pos] = kk[j];
-    if (vout) vout[pos] = vv[j];
+    const uint32_t lpos = dstart[dg[j]] + wpre[w][dg[j]] + lp[j];
+    sk[lpos] = kk[j];
+    if (VALS) sv[lpos] = vv[j];
+  }
+  __syncthreads();
+  const uint32_t cnt_blk = (uint32_t)min((size_t)RS_TILE, n - bbase);
+  for (uint32_t i = threadIdx.x; i < cnt_blk; i += RS_T) {
+    const uint64_t key = sk[i];
+    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const size_t pos = (size_t)offs[(size_t)d * nblk + blockIdx.x] + (i - dstart[d]);
+    kout[pos] = key;
+    if (VALS) vout[pos] = sv[i];
   }
 }
 
@@ -220,8 +251,12 @@ int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* k_alt, uint32_t* v_
     uint32_t* vout = flip ? vals : v_alt;
     hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_T), 0, s, kin, n, shift, hist, nblk);
     scan_exclusive_u32(hist, hist, (size_t)256 * nblk, stmp, s, nullptr);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, kin, vals ? vin : nullptr, kout,
-                       vals ? vout : nullptr, n, shift, hist, nblk);
+    if (vals)
+      hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblk), dim3(RS_T), 0, s, kin, vin, kout, vout, n,
+                         shift, hist, nblk);
+    else
+      hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblk), dim3(RS_T), 0, s, kin, nullptr, kout,
+                         nullptr, n, shift, hist, nblk);
     flip ^= 1;
   }
   return flip;
