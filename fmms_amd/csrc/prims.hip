@@ -114,7 +114,10 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
 
 // ------------------------------------------------------------ radix sort
 constexpr int RS_T = 256;
-constexpr int RS_I = 16;
+#ifndef FMMH_RS_I
+#define FMMH_RS_I 8
+#endif
+constexpr int RS_I = FMMH_RS_I;  // keys per thread of one sort block
 constexpr int RS_TILE = RS_T * RS_I;
 constexpr int RS_WAVE_SPAN = 64 * RS_I;  // consecutive elements per wave
 
