@@ -96,6 +96,8 @@ struct fmmh_ctx {
   Buf c_first, c_count, c_children, c_parent, c_leaf, c_level, c_cx, c_cy, c_cz, c_mass, c_rho,
       c_rope, c_last;
   Buf lvl_cells, leaf_flags, leaf_off, leaf_ids, leaf_of_slot;
+  Buf int_pos, int_cells, lvl_dev;  // internal cells per level (BFS order)
+  std::vector<uint32_t> int_off;   // level L's internal cells: [int_off[L], int_off[L+1])
   Buf scan_tmp, sort_tmp;
   Buf moments, locals, pows, fvals, fmin, fact, binom;
   Buf front_a, front_b, m2l, p2p, ctr;
@@ -118,7 +120,8 @@ struct fmmh_ctx {
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
-                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &scan_tmp, &sort_tmp,
+                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev,
+                  &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
@@ -333,6 +336,7 @@ struct fmmh_ctx {
     launch_leaf_compact(cells(), nc, (uint32_t*)leaf_off.p, (uint32_t*)leaf_ids.p,
                         (uint32_t*)leaf_of_slot.p, s);
     nleaves = read1(tot);
+    internal_lists();
     // geometry: every leaf at once (from its bodies), then internal cells
     // bottom-up level by level (from their children)
     launch_geometry((uint32_t*)leaf_ids.p, nleaves, cells(), bodies(), prm.center_mode, s, 1);
@@ -346,6 +350,28 @@ struct fmmh_ctx {
     sbits = 1;
     while ((1ull << sbits) <= nc) sbits++;
     built = true;
+  }
+
+  // Internal cells of each level, for the per-level upward kernels.
+  void internal_lists() {
+    ensure(int_pos, (size_t)nc * 4);
+    ensure(int_cells, (size_t)nc * 4);
+    launch_internal_flags((uint32_t*)lvl_cells.p, nc, cells(), (uint32_t*)leaf_flags.p, s);
+    uint32_t* itot = nullptr;
+    scan_exclusive_u32((uint32_t*)leaf_flags.p, (uint32_t*)int_pos.p, nc, (uint32_t*)scan_tmp.p, s,
+                       &itot);
+    launch_internal_compact((uint32_t*)lvl_cells.p, nc, cells(), (uint32_t*)int_pos.p,
+                            (uint32_t*)int_cells.p, s);
+    const uint32_t nl = (uint32_t)lvl_off.size();
+    ensure(lvl_dev, (size_t)nl * 8);
+    uint32_t* h = (uint32_t*)host_stage((size_t)nl * 4);
+    std::memcpy(h, lvl_off.data(), (size_t)nl * 4);
+    HCHECK(hipMemcpyAsync(lvl_dev.p, h, (size_t)nl * 4, hipMemcpyHostToDevice, s));
+    launch_level_pick((uint32_t*)int_pos.p, itot, nc, (uint32_t*)lvl_dev.p, nl,
+                      (uint32_t*)lvl_dev.p + nl, s);
+    HCHECK(hipMemcpyAsync(h, (uint32_t*)lvl_dev.p + nl, (size_t)nl * 4, hipMemcpyDeviceToHost, s));
+    HCHECK(hipStreamSynchronize(s));
+    int_off.assign(h, h + nl);
   }
 
   // --------------------------------------------------------------- evaluate
@@ -370,7 +396,7 @@ struct fmmh_ctx {
     if (p <= TEMPLATED_MAX_P) {
       launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
-        launch_m2m(p, (uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, mom, s);
+        launch_m2m(p, (uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, mom, s);
     } else {
       launch_fill_u64(mom, nc * D * 2, 0, s);  // kernel fill: safe under graph capture
       launch_p2m_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
@@ -382,7 +408,7 @@ struct fmmh_ctx {
       double* q = (double*)ctrace.p;
       launch_trace_leaf((uint32_t*)leaf_ids.p, nleaves, c, b, q, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
-        launch_trace_up((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], c, p, mom,
+        launch_trace_up((uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, p, mom,
                         q, s);
     }
   }

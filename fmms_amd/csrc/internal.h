@@ -81,6 +81,12 @@ void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBod
                      int center_mode, hipStream_t s, int which);
 void launch_ropes(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, hipStream_t s);
 void launch_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags, hipStream_t s);
+void launch_internal_flags(const uint32_t* lvl_cells, uint32_t nc, DevCells c, uint32_t* flags,
+                           hipStream_t s);
+void launch_internal_compact(const uint32_t* lvl_cells, uint32_t nc, DevCells c,
+                             const uint32_t* pos, uint32_t* out, hipStream_t s);
+void launch_level_pick(const uint32_t* pos, const uint32_t* total, uint32_t nc, const uint32_t* at,
+                       uint32_t nl, uint32_t* out, hipStream_t s);
 void launch_leaf_compact(DevCells c, uint32_t nc, const uint32_t* off, uint32_t* leaf_ids,
                          uint32_t* leaf_of_slot, hipStream_t s);
 

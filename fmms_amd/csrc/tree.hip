@@ -642,4 +642,49 @@ void launch_leaf_compact(DevCells c, uint32_t nc, const uint32_t* off, uint32_t*
                      leaf_of_slot);
 }
 
+// Internal cells of every level in BFS order (levels stay contiguous):
+// flag, exclusive scan, compact; then each level's start in the compacted
+// list is picked at the level's BFS offset.
+__global__ void __launch_bounds__(TB) k_internal_flags(const uint32_t* __restrict__ lvl_cells,
+                                                      uint32_t nc, DevCells c,
+                                                      uint32_t* __restrict__ flags) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i < nc) flags[i] = c.leaf[lvl_cells[i]] ? 0u : 1u;
+}
+
+__global__ void __launch_bounds__(TB) k_internal_compact(const uint32_t* __restrict__ lvl_cells,
+                                                        uint32_t nc, DevCells c,
+                                                        const uint32_t* __restrict__ pos,
+                                                        uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= nc) return;
+  const uint32_t id = lvl_cells[i];
+  if (!c.leaf[id]) out[pos[i]] = id;
+}
+
+__global__ void k_level_pick(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ total,
+                             uint32_t nc, const uint32_t* __restrict__ at, uint32_t nl,
+                             uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nl) out[i] = at[i] < nc ? pos[at[i]] : *total;
+}
+
+void launch_internal_flags(const uint32_t* lvl_cells, uint32_t nc, DevCells c, uint32_t* flags,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_internal_flags, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, lvl_cells, nc, c,
+                     flags);
+}
+
+void launch_internal_compact(const uint32_t* lvl_cells, uint32_t nc, DevCells c,
+                             const uint32_t* pos, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_internal_compact, dim3((nc + TB - 1) / TB), dim3(TB), 0, s, lvl_cells, nc,
+                     c, pos, out);
+}
+
+void launch_level_pick(const uint32_t* pos, const uint32_t* total, uint32_t nc, const uint32_t* at,
+                       uint32_t nl, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_level_pick, dim3((nl + 63) / 64), dim3(64), 0, s, pos, total, nc, at, nl,
+                     out);
+}
+
 }  // namespace fmmh

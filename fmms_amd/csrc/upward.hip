@@ -105,44 +105,135 @@ __device__ __forceinline__ void upsilon_column_store(double x, double y, double 
   }
 }
 
-constexpr int M2M_T = 128;
+// M2M work schedule. Output coefficient e = (n, m) of a parent is the sum,
+// child by child, of its terms (k, l): Upsilon_k^l(child offset) times the
+// child's M_{n-k}^{m-l}, in the reference's order (k ascending, then l).
+// The schedule lists every output's terms as LDS byte offsets
+// (Upsilon index * 16 | moment index * 16 << 16) and orders the outputs
+// ("slots") by term count, longest first, so that the eight outputs a wave
+// works on at a time have nearly equal trip counts.
+constexpr int m2m_terms(int n, int m) {
+  int t = 0;
+  for (int k = 0; k <= n; k++) {
+    const int nk = n - k;
+    const int lo = -k > m - nk ? -k : m - nk, hi = k < m + nk ? k : m + nk;
+    if (hi >= lo) t += hi - lo + 1;
+  }
+  return t;
+}
+constexpr int m2m_total_terms(int p) {
+  int t = 0;
+  for (int n = 0; n <= p; n++)
+    for (int m = -n; m <= n; m++) t += m2m_terms(n, m);
+  return t;
+}
 
-// M2M (moments.rs:56-90): one block per cell of a level, one thread per
-// output coefficient; children in octant order, the triple sum in the
-// reference's (k, l) order. Leaves of the level exit at once.
+template <int P>
+struct M2mSched {
+  static constexpr int D = hdim(P);
+  static constexpr int T = m2m_total_terms(P);
+  uint32_t code[T];
+  uint16_t e_of[D], base[D], nt[D];
+};
+
+template <int P>
+constexpr M2mSched<P> make_m2m_sched() {
+  M2mSched<P> s{};
+  constexpr int D = M2mSched<P>::D;
+  int cnt[D] = {};
+  int ord[D] = {};
+  for (int e = 0; e < D; e++) {
+    int n = 0;
+    while ((n + 1) * (n + 1) <= e) n++;
+    cnt[e] = m2m_terms(n, e - n * (n + 1));
+    ord[e] = e;
+  }
+  for (int i = 1; i < D; i++) {  // stable insertion sort, longest first
+    const int v = ord[i];
+    int j = i;
+    while (j > 0 && cnt[ord[j - 1]] < cnt[v]) {
+      ord[j] = ord[j - 1];
+      j--;
+    }
+    ord[j] = v;
+  }
+  int at = 0;
+  for (int slot = 0; slot < D; slot++) {
+    const int e = ord[slot];
+    int n = 0;
+    while ((n + 1) * (n + 1) <= e) n++;
+    const int m = e - n * (n + 1);
+    s.e_of[slot] = (uint16_t)e;
+    s.base[slot] = (uint16_t)at;
+    s.nt[slot] = (uint16_t)cnt[e];
+    for (int k = 0; k <= n; k++) {
+      const int nk = n - k;
+      const int lo = -k > m - nk ? -k : m - nk, hi = k < m + nk ? k : m + nk;
+      for (int l = lo; l <= hi; l++)
+        s.code[at++] = (uint32_t)(hidx(k, l) * 16) | ((uint32_t)(hidx(nk, m - l) * 16) << 16);
+    }
+  }
+  return s;
+}
+
+template <int P>
+__device__ constexpr M2mSched<P> kM2mSched = make_m2m_sched<P>();
+
+constexpr int M2M_T = 256;
+
+// M2M (moments.rs:56-90): one block per internal cell of a level. The
+// children's Upsilon tables and moments are staged in LDS; thread (slot
+// group g, child o) computes child o's partial sum of the outputs in slots
+// g, g + 32, g + 64, ... (each from zero, terms in the reference's order),
+// then each output adds its children's partials in octant order. Every
+// output is therefore the reference's sum, bit for bit.
 template <int P>
 __global__ void __launch_bounds__(M2M_T) k_m2m(const uint32_t* __restrict__ cells, uint32_t cnt,
                                               DevCells c, cplx* __restrict__ moments) {
-  constexpr int D = hdim(P);
+  using S = M2mSched<P>;
+  constexpr int D = S::D;
   __shared__ cplx u[8][D];
-  uint32_t id = cells[blockIdx.x];
-  if (c.leaf[id]) return;
+  __shared__ cplx mo[8][D];
+  __shared__ cplx part[8][D];
+  __shared__ uint32_t code[S::T];
+  const uint32_t id = cells[blockIdx.x];
+  if (c.leaf[id]) return;  // block-uniform
   const double zx = c.cx[id], zy = c.cy[id], zz = c.cz[id];
   const int32_t* ch = c.children + (size_t)id * 8;
   for (int t = threadIdx.x; t < 8 * (P + 1); t += M2M_T) {
-    int o = t / (P + 1), m = t % (P + 1);
-    int q = ch[o];
+    const int o = t / (P + 1), m = t % (P + 1);
+    const int q = ch[o];
     if (q < 0) continue;
     upsilon_column_store<P>(c.cx[q] - zx, c.cy[q] - zy, c.cz[q] - zz, m, u[o]);
   }
+  for (int t = threadIdx.x; t < 8 * D; t += M2M_T) {
+    const int o = t / D, e = t - o * D;
+    const int q = ch[o];
+    if (q >= 0) mo[o][e] = moments[(size_t)q * D + e];
+  }
+  for (int t = threadIdx.x; t < S::T; t += M2M_T) code[t] = kM2mSched<P>.code[t];
+  __syncthreads();
+  const int o = threadIdx.x & 7;
+  const bool has = ch[o] >= 0;
+  const char* ub = (const char*)&u[o][0];
+  const char* mb = (const char*)&mo[o][0];
+  for (int slot = threadIdx.x >> 3; slot < D; slot += M2M_T / 8) {
+    const int e = kM2mSched<P>.e_of[slot], b = kM2mSched<P>.base[slot];
+    const int nt = has ? (int)kM2mSched<P>.nt[slot] : 0;
+    cplx acc = cz();
+    for (int t = 0; t < nt; t++) {
+      const uint32_t cd = code[b + t];
+      const cplx uv = *(const cplx*)(ub + (cd & 0xFFFFu));
+      const cplx mv = *(const cplx*)(mb + (cd >> 16));
+      acc = cadd(acc, cmul(uv, mv));
+    }
+    part[o][e] = acc;
+  }
   __syncthreads();
   for (int e = threadIdx.x; e < D; e += M2M_T) {
-    int n = 0;
-    while ((n + 1) * (n + 1) <= e) n++;
-    int m = e - n * (n + 1);
     cplx acc = cz();
-    for (int o = 0; o < 8; o++) {
-      int q = ch[o];
-      if (q < 0) continue;
-      const cplx* mc = moments + (size_t)q * D;
-      cplx s = cz();
-      for (int k = 0; k <= n; k++) {
-        int nk = n - k;
-        int lo = max(-k, m - nk), hi = min(k, m + nk);
-        for (int l = lo; l <= hi; l++) s = cadd(s, cmul(u[o][hidx(k, l)], mc[hidx(nk, m - l)]));
-      }
-      acc = cadd(acc, s);
-    }
+    for (int q = 0; q < 8; q++)
+      if (ch[q] >= 0) acc = cadd(acc, part[q][e]);
     moments[(size_t)id * D + e] = acc;
   }
 }
