@@ -97,6 +97,8 @@ struct fmmh_ctx {
       c_rope, c_last;
   Buf lvl_cells, leaf_flags, leaf_off, leaf_ids, leaf_of_slot;
   Buf int_pos, int_cells, lvl_dev;  // internal cells per level (BFS order)
+  Buf split_bnd;                    // octant boundaries of a level's cells (9 words each)
+  Buf leaf_bc, lbc_keys, lbc_keys_alt;  // leaves ordered by body count
   std::vector<uint32_t> int_off;   // level L's internal cells: [int_off[L], int_off[L+1])
   Buf scan_tmp, sort_tmp;
   Buf moments, locals, pows, fvals, fmin, fact, binom;
@@ -120,7 +122,7 @@ struct fmmh_ctx {
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
-                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev,
+                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev, &split_bnd, &leaf_bc, &lbc_keys, &lbc_keys_alt,
                   &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
@@ -290,9 +292,10 @@ struct fmmh_ctx {
       uint32_t s0 = lvl_off[L], cnt = lvl_off[L + 1] - s0;
       ensure(nchild, (size_t)cnt * 4);
       ensure(child_off, (size_t)cnt * 4);
+      ensure(split_bnd, (size_t)cnt * 36);
       ensure_scan(cnt);
       launch_split_count(bfs(), s0, cnt, keys_sorted, (uint32_t)prm.n_leaf, vlev,
-                         (uint32_t*)nchild.p, s);
+                         (uint32_t*)nchild.p, (uint32_t*)split_bnd.p, s);
       uint32_t* tot = nullptr;
       scan_exclusive_u32((uint32_t*)nchild.p, (uint32_t*)child_off.p, cnt, (uint32_t*)scan_tmp.p, s,
                          &tot);
@@ -300,8 +303,9 @@ struct fmmh_ctx {
       if (total == 0) break;
       size_t need = (size_t)lvl_off[L + 1] + total;
       if (need > bfs_first.cap / 4) ensure_bfs(need + need / 2, true);
-      launch_split_emit(bfs(), s0, cnt, lvl_off[L + 1], keys_sorted, (uint32_t)prm.n_leaf, vlev,
-                        (uint32_t*)child_off.p, (uint32_t*)chain.p, (uint8_t*)top.p, s);
+      launch_split_emit(bfs(), s0, cnt, lvl_off[L + 1], (uint32_t)prm.n_leaf, vlev,
+                        (uint32_t*)split_bnd.p, (uint32_t*)child_off.p, (uint32_t*)chain.p,
+                        (uint8_t*)top.p, s);
       lvl_off.push_back((uint32_t)need);
     }
     nc = lvl_off.back();
@@ -337,9 +341,16 @@ struct fmmh_ctx {
                         (uint32_t*)leaf_of_slot.p, s);
     nleaves = read1(tot);
     internal_lists();
+    ensure(leaf_bc, (size_t)nleaves * 4 + 4);
+    ensure(lbc_keys, (size_t)nleaves * 8 + 8);
+    ensure(lbc_keys_alt, (size_t)nleaves * 8 + 8);
+    ensure(sort_tmp, radix_tmp_words(nleaves) * 4 + 64);
+    launch_leaves_by_count((uint32_t*)leaf_ids.p, nleaves, cells(), (uint64_t*)lbc_keys.p,
+                           (uint64_t*)lbc_keys_alt.p, (uint32_t*)sort_tmp.p, (uint32_t*)leaf_bc.p,
+                           s);
     // geometry: every leaf at once (from its bodies), then internal cells
     // bottom-up level by level (from their children)
-    launch_geometry((uint32_t*)leaf_ids.p, nleaves, cells(), bodies(), prm.center_mode, s, 1);
+    launch_geometry_leaves((uint32_t*)leaf_ids.p, nleaves, cells(), bodies(), prm.center_mode, s);
     for (int L = (int)max_level - 1; L >= 0; L--)
       launch_geometry((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(),
                       bodies(), prm.center_mode, s, 2);
@@ -394,7 +405,7 @@ struct fmmh_ctx {
     mark(ST_UPWARD);
     cplx* mom = (cplx*)moments.p;
     if (p <= TEMPLATED_MAX_P) {
-      launch_p2m(p, (uint32_t*)leaf_ids.p, nleaves, c, b, mom, s);
+      launch_p2m(p, (uint32_t*)leaf_bc.p, nleaves, c, b, mom, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
         launch_m2m(p, (uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, mom, s);
     } else {

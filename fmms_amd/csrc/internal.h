@@ -68,19 +68,25 @@ void launch_gather(const double* pos, const double* mass, const uint32_t* order,
                    DevBodies b, hipStream_t s);
 void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipStream_t s);
 void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
-                        uint32_t n_leaf, int vlev, uint32_t* nchild, hipStream_t s);
-void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
-                       uint32_t n_leaf, int vlev, const uint32_t* child_off, uint32_t* chain,
-                       uint8_t* top, hipStream_t s);
+                        uint32_t n_leaf, int vlev, uint32_t* nchild, uint32_t* bounds,
+                        hipStream_t s);
+void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, uint32_t n_leaf,
+                       int vlev, const uint32_t* bounds, const uint32_t* child_off,
+                       uint32_t* chain, uint8_t* top, hipStream_t s);
 void launch_dfs_ids(DevBfs bfs, uint32_t nc, const uint32_t* cstart, const uint8_t* top,
                     hipStream_t s);
 void launch_scatter_cells(DevBfs bfs, uint32_t nc, uint32_t n_leaf, int vlev, DevCells c,
                           uint32_t* lvl_cells, hipStream_t s);
 // which = 0: all cells of the list, 1: its leaves only, 2: its internal cells only.
+void launch_geometry_leaves(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                            int center_mode, hipStream_t s);
 void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
                      int center_mode, hipStream_t s, int which);
 void launch_ropes(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, hipStream_t s);
 void launch_leaf_flags(DevCells c, uint32_t nc, uint32_t* flags, hipStream_t s);
+void launch_leaves_by_count(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c,
+                            uint64_t* keys, uint64_t* keys_alt, uint32_t* tmp, uint32_t* out,
+                            hipStream_t s);
 void launch_internal_flags(const uint32_t* lvl_cells, uint32_t nc, DevCells c, uint32_t* flags,
                            hipStream_t s);
 void launch_internal_compact(const uint32_t* lvl_cells, uint32_t nc, DevCells c,

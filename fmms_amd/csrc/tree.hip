@@ -263,74 +263,110 @@ void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipSt
   hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, bfs, (uint32_t)n, chain, top);
 }
 
-// First index in [lo, hi) whose octant digit is >= oct (digits ascend there).
-__device__ __forceinline__ uint32_t digit_lb(const uint64_t* __restrict__ keys, uint32_t lo,
-                                             uint32_t hi, int shift, uint32_t oct) {
-  while (lo < hi) {
-    uint32_t mid = lo + ((hi - lo) >> 1);
-    if (((keys[mid] >> shift) & 7u) < oct)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// Octant boundaries of an internal cell (tree.rs:420-443): bnd[o]..bnd[o+1].
-__device__ __forceinline__ void octant_bounds(const uint64_t* __restrict__ keys, uint32_t f,
-                                              uint32_t c, int level, uint32_t bnd[9]) {
-  const int shift = 3 * (BITS - (level + 1));
-  bnd[0] = f;
-  bnd[8] = f + c;
-#pragma unroll
-  for (uint32_t o = 1; o < 8; o++) bnd[o] = digit_lb(keys, bnd[o - 1], f + c, shift, o);
-}
-
 // tree.rs:414 leaf rule; the virtual levels of an ensemble (level < vlev,
 // cells holding several systems) always split.
 __device__ __forceinline__ bool is_internal(uint32_t count, int level, uint32_t n_leaf, int vlev) {
   return level < vlev || !(count <= n_leaf || level >= BITS);
 }
 
-__global__ void __launch_bounds__(TB) k_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt,
-                                                   const uint64_t* __restrict__ keys,
-                                                   uint32_t n_leaf, int vlev,
-                                                   uint32_t* __restrict__ nchild) {
-  uint32_t i = blockIdx.x * TB + threadIdx.x;
+// Octant boundaries of an internal cell (tree.rs:420-443): bnd[o] is the
+// first slot whose octant digit at this level is >= o, so the children are
+// bnd[o]..bnd[o+1]. One wave finds them by a 65-ary search per boundary
+// (64 probes a round, all seven boundaries at once): a cell of N bodies
+// takes about log65(N) rounds of loads instead of 7 log2(N) dependent ones.
+__device__ __forceinline__ void octant_bounds_wave(const uint64_t* __restrict__ keys, uint32_t f,
+                                                   uint32_t c, int level, int lane,
+                                                   uint32_t bnd[9]) {
+  const int shift = 3 * (BITS - (level + 1));
+  uint32_t lo[8], hi[8];
+#pragma unroll
+  for (int o = 1; o < 8; o++) {
+    lo[o] = f;
+    hi[o] = f + c;
+  }
+  // digits are non-decreasing over the cell's slots, so every ballot below
+  // is a prefix of the wave; a round shrinks a bracket of length L > 64 to
+  // at most L/65 + 1 and settles one of length <= 64 (7 rounds cover 2^32)
+  for (int it = 0; it < 8; it++) {
+    bool more = false;
+#pragma unroll
+    for (int o = 1; o < 8; o++) {
+      const uint32_t len = hi[o] - lo[o];
+      if (len == 0) continue;
+      more = true;
+      if (len <= 64) {
+        const bool v = (uint32_t)lane < len;
+        const uint32_t d = v ? (uint32_t)((keys[lo[o] + (uint32_t)lane] >> shift) & 7u) : 8u;
+        lo[o] += (uint32_t)__popcll(__ballot(d < (uint32_t)o));
+        hi[o] = lo[o];
+      } else {
+        const uint32_t q = lo[o] + (uint32_t)(((uint64_t)(lane + 1) * len) / 65u);
+        const uint32_t d = (uint32_t)((keys[q] >> shift) & 7u);
+        const uint32_t t = (uint32_t)__popcll(__ballot(d < (uint32_t)o));
+        const uint32_t nlo = t == 0 ? lo[o] : lo[o] + (uint32_t)(((uint64_t)t * len) / 65u) + 1u;
+        const uint32_t nhi = t == 64 ? hi[o] : lo[o] + (uint32_t)(((uint64_t)(t + 1) * len) / 65u);
+        lo[o] = nlo;
+        hi[o] = nhi;
+      }
+    }
+    if (!more) break;
+  }
+  bnd[0] = f;
+#pragma unroll
+  for (int o = 1; o < 8; o++) bnd[o] = lo[o];
+  bnd[8] = f + c;
+}
+
+constexpr int SPLIT_WAVES = 4;
+
+// One wave per cell of a level: child count and octant boundaries, which
+// are kept (9 words a cell) for k_split_emit.
+__global__ void __launch_bounds__(64 * SPLIT_WAVES) k_split_count(DevBfs bfs, uint32_t s0,
+                                                                uint32_t cnt,
+                                                                const uint64_t* __restrict__ keys,
+                                                                uint32_t n_leaf, int vlev,
+                                                                uint32_t* __restrict__ nchild,
+                                                                uint32_t* __restrict__ bounds) {
+  const uint32_t i = blockIdx.x * SPLIT_WAVES + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= cnt) return;
-  uint32_t b = s0 + i;
-  uint32_t f = bfs.first[b], c = bfs.count[b];
-  int lvl = bfs.level[b];
+  const uint32_t b = s0 + i;
+  const uint32_t f = bfs.first[b], c = bfs.count[b];
+  const int lvl = bfs.level[b];
   uint32_t k = 0;
   if (is_internal(c, lvl, n_leaf, vlev)) {
     uint32_t bnd[9];
-    octant_bounds(keys, f, c, lvl, bnd);
+    octant_bounds_wave(keys, f, c, lvl, lane, bnd);
 #pragma unroll
     for (int o = 0; o < 8; o++) k += bnd[o + 1] > bnd[o] ? 1u : 0u;
+#pragma unroll
+    for (int o = 0; o < 9; o++)
+      if (lane == o) bounds[(size_t)i * 9 + o] = bnd[o];
   }
-  nchild[i] = k;
+  if (lane == 0) nchild[i] = k;
 }
 
 void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
-                        uint32_t n_leaf, int vlev, uint32_t* nchild, hipStream_t s) {
-  hipLaunchKernelGGL(k_split_count, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, keys,
-                     n_leaf, vlev, nchild);
+                        uint32_t n_leaf, int vlev, uint32_t* nchild, uint32_t* bounds,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_split_count, dim3((cnt + SPLIT_WAVES - 1) / SPLIT_WAVES),
+                     dim3(64 * SPLIT_WAVES), 0, s, bfs, s0, cnt, keys, n_leaf, vlev, nchild,
+                     bounds);
 }
 
 __global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt,
-                                                  uint32_t e0, const uint64_t* __restrict__ keys,
-                                                  uint32_t n_leaf, int vlev,
+                                                  uint32_t e0, uint32_t n_leaf, int vlev,
+                                                  const uint32_t* __restrict__ bounds,
                                                   const uint32_t* __restrict__ child_off,
                                                   uint32_t* __restrict__ chain,
                                                   uint8_t* __restrict__ top) {
   uint32_t i = blockIdx.x * TB + threadIdx.x;
   if (i >= cnt) return;
   uint32_t b = s0 + i;
-  uint32_t f = bfs.first[b], c = bfs.count[b];
+  uint32_t c = bfs.count[b];
   int lvl = bfs.level[b];
   if (!is_internal(c, lvl, n_leaf, vlev)) return;
-  uint32_t bnd[9];
-  octant_bounds(keys, f, c, lvl, bnd);
+  const uint32_t* bnd = bounds + (size_t)i * 9;
   uint32_t w = e0 + child_off[i];
   for (int o = 0; o < 8; o++) {
     uint32_t lo = bnd[o], hi = bnd[o + 1];
@@ -348,11 +384,11 @@ __global__ void __launch_bounds__(TB) k_split_emit(DevBfs bfs, uint32_t s0, uint
   }
 }
 
-void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, const uint64_t* keys,
-                       uint32_t n_leaf, int vlev, const uint32_t* child_off, uint32_t* chain,
-                       uint8_t* top, hipStream_t s) {
+void launch_split_emit(DevBfs bfs, uint32_t s0, uint32_t cnt, uint32_t e0, uint32_t n_leaf,
+                       int vlev, const uint32_t* bounds, const uint32_t* child_off,
+                       uint32_t* chain, uint8_t* top, hipStream_t s) {
   hipLaunchKernelGGL(k_split_emit, dim3((cnt + TB - 1) / TB), dim3(TB), 0, s, bfs, s0, cnt, e0,
-                     keys, n_leaf, vlev, child_off, chain, top);
+                     n_leaf, vlev, bounds, child_off, chain, top);
 }
 
 __global__ void __launch_bounds__(TB) k_dfs_ids(DevBfs bfs, uint32_t nc,
@@ -581,6 +617,182 @@ __global__ void __launch_bounds__(TB) k_geometry(const uint32_t* __restrict__ ce
   c.rho[id] = rho;
 }
 
+// Leaf geometry with one wave per leaf: the same values as k_geometry's
+// leaf branch (tree.rs:152-227, 257-275, 290-389). Order-dependent sums
+// (mass, centre of mass) run in slot order over bodies read from the lanes
+// that hold them; the farthest-point searches, the box and the radius and
+// rho maxima spread over the lanes (a maximum, ties to the lowest slot, does
+// not depend on the order); the grow passes test a chunk of bodies against
+// the current sphere at once and apply the first failing body's update, then
+// test the bodies after it again, which is exactly the sequential pass.
+constexpr int GEO_WAVES = 4;
+
+// Lane k's value (k wave-uniform) in every lane.
+__device__ __forceinline__ double lane_read(double v, uint32_t k) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)k);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double wave_max_pos(double v) {  // v >= +0 in every lane
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Farthest body from slot a (first slot on ties), over the leaf's bodies.
+__device__ __forceinline__ uint32_t wave_farthest(DevBodies b, uint32_t f, uint32_t n, uint32_t a,
+                                                  int lane) {
+  const double ax = b.x[f + a], ay = b.y[f + a], az = b.z[f + a];
+  double best = -1.0;
+  uint32_t bi = 0;
+  for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+    const double dx = b.x[f + i] - ax, dy = b.y[f + i] - ay, dz = b.z[f + i] - az;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > best) {
+      best = d2;
+      bi = i;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const uint32_t oi = (uint32_t)__shfl_xor((int)bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  return bi;
+}
+
+__global__ void __launch_bounds__(64 * GEO_WAVES) k_geometry_leaf(
+    const uint32_t* __restrict__ leaf_ids, uint32_t nleaves, DevCells c, DevBodies b, int mode) {
+  const uint32_t li = blockIdx.x * GEO_WAVES + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (li >= nleaves) return;
+  const uint32_t id = leaf_ids[li];
+  const uint32_t f = c.first[id], n = c.count[id];
+  // slot-ordered sums over chunks of 64 bodies held one per lane
+  double ms = 0.0, com[3] = {0.0, 0.0, 0.0};
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t i = base + (uint32_t)lane;
+    const bool have = i < n;
+    const double mi = have ? b.m[f + i] : 0.0;
+    const uint32_t cnt = min(64u, n - base);
+    if (mode == 0) {  // com_of_slots, tree.rs:257-275
+      const double xi = have ? b.x[f + i] : 0.0, yi = have ? b.y[f + i] : 0.0,
+                   zi = have ? b.z[f + i] : 0.0;
+      for (uint32_t k = 0; k < cnt; k++) {
+        const double m = lane_read(mi, k);
+        ms += m;
+        com[0] += m * lane_read(xi, k);
+        com[1] += m * lane_read(yi, k);
+        com[2] += m * lane_read(zi, k);
+      }
+    } else {
+      for (uint32_t k = 0; k < cnt; k++) ms += lane_read(mi, k);
+    }
+  }
+  double cen[3];
+  if (mode == 0) {
+    cen[0] = com[0] / ms;
+    cen[1] = com[1] / ms;
+    cen[2] = com[2] / ms;
+  } else {  // Ritter sphere, tree.rs:290-389 (bodies have radius 0)
+    const double MINP = 2.2250738585072014e-308;
+    const double rad = 0.0;
+    const uint32_t a = wave_farthest(b, f, n, 0, lane);
+    const uint32_t bb = wave_farthest(b, f, n, a, lane);
+    const double pa[3] = {b.x[f + a], b.y[f + a], b.z[f + a]};
+    const double pb[3] = {b.x[f + bb], b.y[f + bb], b.z[f + bb]};
+    double dir[3];
+    {
+      const double dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+      const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), MINP);
+      dir[0] = dx / d;
+      dir[1] = dy / d;
+      dir[2] = dz / d;
+    }
+    const double fa[3] = {pa[0] + rad * dir[0], pa[1] + rad * dir[1], pa[2] + rad * dir[2]};
+    const double fb[3] = {pb[0] - rad * dir[0], pb[1] - rad * dir[1], pb[2] - rad * dir[2]};
+    double c0[3] = {(fa[0] + fb[0]) * 0.5, (fa[1] + fb[1]) * 0.5, (fa[2] + fb[2]) * 0.5};
+    double r0 = dist3(fa[0], fa[1], fa[2], fb[0], fb[1], fb[2]) * 0.5;
+    for (int pass = 0; pass < 2; pass++) {
+      for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t i = base + (uint32_t)lane;
+        const bool have = i < n;
+        const double qx = have ? b.x[f + i] : 0.0, qy = have ? b.y[f + i] : 0.0,
+                     qz = have ? b.z[f + i] : 0.0;
+        int start = 0;
+        // each round either ends the chunk or moves start past one lane
+        while (start < 64) {
+          double d = 0.0;
+          bool out = false;
+          if (have && lane >= start) {
+            d = dist3(qx, qy, qz, c0[0], c0[1], c0[2]);
+            out = d + rad > r0;
+          }
+          const uint64_t m = __ballot(out);
+          if (m == 0) break;
+          const int j = __ffsll((long long)m) - 1;
+          const double q[3] = {__shfl(qx, j, 64), __shfl(qy, j, 64), __shfl(qz, j, 64)};
+          const double dj = __shfl(d, j, 64);
+          const double dd = fmax(dj, MINP);
+          const double dr[3] = {(q[0] - c0[0]) / dd, (q[1] - c0[1]) / dd, (q[2] - c0[2]) / dd};
+          const double pnt[3] = {q[0] + rad * dr[0], q[1] + rad * dr[1], q[2] + rad * dr[2]};
+          const double d_far = dj + rad;
+          const double t = (d_far - r0) / (2.0 * d_far);
+          for (int k = 0; k < 3; k++) c0[k] += t * (pnt[k] - c0[k]);
+          r0 = (d_far + r0) * 0.5;
+          start = j + 1;
+        }
+      }
+    }
+    // the box: a min/max does not depend on the order (up to the sign of a
+    // zero extreme, which changes no centre unless both extremes are zero)
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+      const double q[3] = {b.x[f + i], b.y[f + i], b.z[f + i]};
+      for (int d = 0; d < 3; d++) {
+        lo[d] = fmin(lo[d], q[d] - rad);
+        hi[d] = fmax(hi[d], q[d] + rad);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+        hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+      }
+    const double cb[3] = {(lo[0] + hi[0]) * 0.5, (lo[1] + hi[1]) * 0.5, (lo[2] + hi[2]) * 0.5};
+    double rb = 0.0;
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64)
+      rb = fmax(rb, dist3(b.x[f + i], b.y[f + i], b.z[f + i], cb[0], cb[1], cb[2]) + rad);
+    rb = wave_max_pos(rb);
+    for (int k = 0; k < 3; k++) cen[k] = rb < r0 ? cb[k] : c0[k];
+  }
+  double rho = 0.0;
+  for (uint32_t i = (uint32_t)lane; i < n; i += 64)
+    rho = fmax(rho, dist3(b.x[f + i], b.y[f + i], b.z[f + i], cen[0], cen[1], cen[2]));
+  rho = wave_max_pos(rho);
+  if (lane == 0) {
+    c.mass[id] = ms;
+    c.cx[id] = cen[0];
+    c.cy[id] = cen[1];
+    c.cz[id] = cen[2];
+    c.rho[id] = rho;
+  }
+}
+
+void launch_geometry_leaves(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                            int center_mode, hipStream_t s) {
+  if (nleaves == 0) return;
+  hipLaunchKernelGGL(k_geometry_leaf, dim3((nleaves + GEO_WAVES - 1) / GEO_WAVES),
+                     dim3(64 * GEO_WAVES), 0, s, leaf_ids, nleaves, c, b, center_mode);
+}
+
 void launch_geometry(const uint32_t* lvl_cells, uint32_t cnt, DevCells c, DevBodies b,
                      int center_mode, hipStream_t s, int which) {
   if (cnt == 0) return;
@@ -667,6 +879,34 @@ __global__ void k_level_pick(const uint32_t* __restrict__ pos, const uint32_t* _
                              uint32_t* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nl) out[i] = at[i] < nc ? pos[at[i]] : *total;
+}
+
+// Leaves ordered by body count (for per-leaf kernels whose waves should see
+// leaves of similar size): key = min(count, 65535) << 32 | leaf id, sorted
+// on bits 32..47, then the ids read back.
+__global__ void __launch_bounds__(TB) k_leaf_count_keys(const uint32_t* __restrict__ leaf_ids,
+                                                       uint32_t nleaves, DevCells c,
+                                                       uint64_t* __restrict__ keys) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= nleaves) return;
+  const uint32_t id = leaf_ids[i];
+  keys[i] = ((uint64_t)min(c.count[id], 65535u) << 32) | id;
+}
+
+__global__ void __launch_bounds__(TB) k_low32(const uint64_t* __restrict__ keys, uint32_t n,
+                                             uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)keys[i];
+}
+
+void launch_leaves_by_count(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c,
+                            uint64_t* keys, uint64_t* keys_alt, uint32_t* tmp, uint32_t* out,
+                            hipStream_t s) {
+  if (nleaves == 0) return;
+  const dim3 g((nleaves + TB - 1) / TB);
+  hipLaunchKernelGGL(k_leaf_count_keys, g, dim3(TB), 0, s, leaf_ids, nleaves, c, keys);
+  const int fl = radix_sort_u64(keys, nullptr, keys_alt, nullptr, nleaves, 32, 48, tmp, s);
+  hipLaunchKernelGGL(k_low32, g, dim3(TB), 0, s, fl ? keys_alt : keys, nleaves, out);
 }
 
 void launch_internal_flags(const uint32_t* lvl_cells, uint32_t nc, DevCells c, uint32_t* flags,

@@ -44,17 +44,11 @@ __device__ __forceinline__ void upsilon_column_acc(double x, double y, double z,
   }
 }
 
-// P2M (moments.rs:30-53): one thread per (leaf, m >= 0); bodies in slot order.
-template <int P>
-__global__ void __launch_bounds__(256) k_p2m(const uint32_t* __restrict__ leaf_ids,
-                                            uint32_t nleaves, DevCells c, DevBodies b,
-                                            cplx* __restrict__ moments) {
+// P2M (moments.rs:30-53) of column M of one leaf; bodies in slot order.
+template <int P, int M>
+__device__ __forceinline__ void p2m_column(uint32_t id, DevCells c, DevBodies b,
+                                           cplx* __restrict__ moments) {
   constexpr int D = hdim(P);
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t li = t / (P + 1);
-  int m = (int)(t % (P + 1));
-  if (li >= nleaves) return;
-  uint32_t id = leaf_ids[li];
   const double zx = c.cx[id], zy = c.cy[id], zz = c.cz[id];
   cplx acc[P + 1], accn[P + 1];
 #pragma unroll
@@ -63,15 +57,38 @@ __global__ void __launch_bounds__(256) k_p2m(const uint32_t* __restrict__ leaf_i
   for (uint32_t k = f; k < e; k++) {
     double x = b.x[k] - zx, y = b.y[k] - zy, z = b.z[k] - zz;
     double r2 = x * x + y * y + z * z;
-    upsilon_column_acc<P>(x, y, z, r2, m, b.m[k], acc, accn);
+    upsilon_column_acc<P>(x, y, z, r2, M, b.m[k], acc, accn);
   }
   cplx* out = moments + (size_t)id * D;
 #pragma unroll
-  for (int n = 0; n <= P; n++) {
-    if (n < m) continue;
-    out[hidx(n, m)] = acc[n];
-    if (m > 0) out[hidx(n, -m)] = accn[n];
+  for (int n = M; n <= P; n++) {
+    out[hidx(n, M)] = acc[n];
+    if (M > 0) out[hidx(n, -M)] = accn[n];
   }
+}
+
+template <int P, int M = 0>
+__device__ __forceinline__ void p2m_dispatch(int m, uint32_t id, DevCells c, DevBodies b,
+                                             cplx* __restrict__ moments) {
+  if constexpr (M <= P) {
+    if (m == M)
+      p2m_column<P, M>(id, c, b, moments);
+    else
+      p2m_dispatch<P, M + 1>(m, id, c, b, moments);
+  }
+}
+
+// One block per 64 leaves, wave m computing column m of each: a wave runs
+// one column's exact work, and the leaves come in body-count order, so its
+// lanes loop over nearly equal numbers of bodies.
+template <int P>
+__global__ void __launch_bounds__(64 * (P + 1)) k_p2m(const uint32_t* __restrict__ leaf_ids,
+                                                     uint32_t nleaves, DevCells c, DevBodies b,
+                                                     cplx* __restrict__ moments) {
+  const int m = (int)(threadIdx.x >> 6);
+  const uint32_t li = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (li >= nleaves) return;
+  p2m_dispatch<P>(m, leaf_ids[li], c, b, moments);
 }
 
 // Full Upsilon column m (both signs) written into a table (for M2M's
@@ -241,9 +258,8 @@ __global__ void __launch_bounds__(M2M_T) k_m2m(const uint32_t* __restrict__ cell
 template <int P>
 static void p2m_t(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
                   cplx* moments, hipStream_t s) {
-  size_t threads = (size_t)nleaves * (P + 1);
-  hipLaunchKernelGGL(k_p2m<P>, dim3((threads + 255) / 256), dim3(256), 0, s, leaf_ids, nleaves, c,
-                     b, moments);
+  hipLaunchKernelGGL(k_p2m<P>, dim3((nleaves + 63) / 64), dim3(64 * (P + 1)), 0, s, leaf_ids,
+                     nleaves, c, b, moments);
 }
 
 template <int P>
