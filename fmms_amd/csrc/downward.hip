@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(64) k_l2l(const uint32_t* __restrict__ cells, 
     }
 }
 
-// L2P, one thread per body (slot): far-field acceleration and potential.
+// L2P, one thread per body (slot): far-field acceleration and potential of
+// the local expansion (the softened trace terms are added by k_l2p_trace).
 template <int P>
 __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restrict__ leaf_of_slot,
                                            const double* __restrict__ cx,
@@ -147,8 +148,6 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
                                            const double* __restrict__ by,
                                            const double* __restrict__ bz,
                                            const cplx* __restrict__ locals,
-                                           const double* __restrict__ lap,
-                                           const int32_t* __restrict__ parent,
                                            double4* __restrict__ far) {
   constexpr int D = hdim(P), NF = hnpos(P);
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,9 +191,7 @@ __global__ void __launch_bounds__(64) k_l2p(uint32_t n, const uint32_t* __restri
       U1[q] = U[q];
     }
   }
-  double4 o = make_double4(-p11r, -p11i, -p10r, p00);
-  if (lap) l2p_trace_walk(lap, parent, cx, cy, cz, c, bx[k], by[k], bz[k], o);  // softened far field
-  far[k] = o;
+  far[k] = make_double4(-p11r, -p11i, -p10r, p00);
 }
 
 template <int P>
@@ -205,9 +202,9 @@ static void l2l_t(const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
 
 template <int P>
 static void l2p_t(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                  const cplx* locals, const double* lap, double4* far, hipStream_t s) {
+                  const cplx* locals, double4* far, hipStream_t s) {
   hipLaunchKernelGGL(k_l2p<P>, dim3((n + 63) / 64), dim3(64), 0, s, n, leaf_of_slot, c.cx, c.cy,
-                     c.cz, b.x, b.y, b.z, locals, lap, c.parent, far);
+                     c.cz, b.x, b.y, b.z, locals, far);
 }
 
 #define FMMH_P_SWITCH2(p, CALL) \
@@ -234,12 +231,37 @@ bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* lo
   return true;
 }
 
+// Softened far field's trace polynomial of every body's leaf and ancestors,
+// added to the body's L2P result (the same additions, in the same order, as
+// inside k_l2p; a kernel of its own keeps the ancestor walk's dependent loads
+// away from k_l2p's register-heavy harmonic sum, so many more waves hide them).
+__global__ void __launch_bounds__(256) k_l2p_trace(uint32_t n,
+                                                  const uint32_t* __restrict__ leaf_of_slot,
+                                                  const double* __restrict__ cx,
+                                                  const double* __restrict__ cy,
+                                                  const double* __restrict__ cz,
+                                                  const double* __restrict__ bx,
+                                                  const double* __restrict__ by,
+                                                  const double* __restrict__ bz,
+                                                  const double* __restrict__ lap,
+                                                  const int32_t* __restrict__ parent,
+                                                  double4* __restrict__ far) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  double4 o = far[k];
+  l2p_trace_walk(lap, parent, cx, cy, cz, leaf_of_slot[k], bx[k], by[k], bz[k], o);
+  far[k] = o;
+}
+
 bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
                 const cplx* locals, const double* lap, double4* far, hipStream_t s) {
   if (n == 0) return true;
-#define CALL(P) l2p_t<P>(n, leaf_of_slot, c, b, locals, lap, far, s)
+#define CALL(P) l2p_t<P>(n, leaf_of_slot, c, b, locals, far, s)
   FMMH_P_SWITCH2(p, CALL)
 #undef CALL
+  if (lap)
+    hipLaunchKernelGGL(k_l2p_trace, dim3((n + 255) / 256), dim3(256), 0, s, n, leaf_of_slot, c.cx,
+                       c.cy, c.cz, b.x, b.y, b.z, lap, c.parent, far);
   return true;
 }
 

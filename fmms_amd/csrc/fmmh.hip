@@ -99,6 +99,7 @@ struct fmmh_ctx {
   Buf int_pos, int_cells, lvl_dev;  // internal cells per level (BFS order)
   Buf split_bnd;                    // octant boundaries of a level's cells (9 words each)
   Buf leaf_bc, lbc_keys, lbc_keys_alt;  // leaves ordered by body count
+  Buf cpm;                              // {cx, cy, cz, mass} per cell
   std::vector<uint32_t> int_off;   // level L's internal cells: [int_off[L], int_off[L+1])
   Buf scan_tmp, sort_tmp;
   Buf moments, locals, pows, fvals, fmin, fact, binom;
@@ -122,7 +123,7 @@ struct fmmh_ctx {
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
-                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev, &split_bnd, &leaf_bc, &lbc_keys, &lbc_keys_alt,
+                  &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev, &split_bnd, &leaf_bc, &lbc_keys, &lbc_keys_alt, &cpm,
                   &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
@@ -357,7 +358,8 @@ struct fmmh_ctx {
     for (int L = 0; L <= (int)max_level; L++)
       launch_ropes((uint32_t*)lvl_cells.p + lvl_off[L], lvl_off[L + 1] - lvl_off[L], cells(), s);
     ensure(walkrec, (size_t)nc * walk_rec_bytes());
-    launch_pack_walk(nc, cells(), walkrec.p, s);
+    ensure(cpm, (size_t)nc * sizeof(double4));
+    launch_pack_walk(nc, cells(), walkrec.p, (double4*)cpm.p, s);
     sbits = 1;
     while ((1ull << sbits) <= nc) sbits++;
     built = true;
@@ -562,8 +564,8 @@ struct fmmh_ctx {
     double* lap = nullptr;
     if (soft2 > 0.0) {
       lap = (double*)clap.p;
-      launch_m2l_trace(nc, (const uint32_t*)tptr.p, msorted, sbits, c, (const double*)ctrace.p,
-                       soft2, p, loc, lap, s);
+      launch_m2l_trace(nc, (const uint32_t*)tptr.p, msorted, sbits, (const double4*)cpm.p,
+                       (const double*)ctrace.p, soft2, p, loc, lap, s);
     }
     mark(ST_L2L);
     for (int L = 1; L <= (int)max_level; L++) {

@@ -128,7 +128,8 @@ struct MacParams {
 };
 size_t walk_rec_bytes();
 void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s);
-void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s);
+// Walk records, and {cx, cy, cz, mass} of every cell as one 32-byte record.
+void launch_pack_walk(uint32_t nc, DevCells c, void* rec, double4* cpm, hipStream_t s);
 void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
                        uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
                        const void* rec, const double* mass, MacParams mac, hipStream_t s);
@@ -194,8 +195,8 @@ void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, D
 void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, const cplx* mom,
                      double* q, hipStream_t s);
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
-                      DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
-                      hipStream_t s);
+                      const double4* cpm, const double* q, double soft2, int p, cplx* locals,
+                      double* lap, hipStream_t s);
 // Target-side trace polynomial of the softened far field, summed over a
 // leaf and all its ancestors (each cell's L, G, H about its own centre; body
 // at (bx, by, bz)): pot += |d|^2 L/6 + |d|^2 (d.G)/10 + |d|^2 (d.H.d)/28

@@ -74,6 +74,8 @@ __global__ void __launch_bounds__(256) k_m2l_emit(const uint32_t* __restrict__ t
   for (uint32_t j = b; j < e; j += M2L_CH) chunks[w++] = M2lChunk{t, j, min(e, j + M2L_CH)};
 }
 
+constexpr int RED_AHEAD = 8;  // chunk partials loaded ahead of the in-order sum
+
 // locals[t] = sum over the target's chunk partials in chunk order; both
 // halves of m written (F_n^{-m} = (-1)^m conj(F_n^m)); empty targets get 0.
 template <int P>
@@ -89,7 +91,19 @@ __global__ void __launch_bounds__(256) k_m2l_reduce(uint32_t nc, const uint32_t*
   while ((n + 1) * (n + 2) / 2 <= e) n++;
   int m = e - n * (n + 1) / 2;
   cplx s = cplx{0.0, 0.0};
-  for (uint32_t c = choff[t]; c < choff[t + 1]; c++) {
+  const uint32_t c0 = choff[t], c1 = choff[t + 1];
+  uint32_t c = c0;
+  for (; c + RED_AHEAD <= c1; c += RED_AHEAD) {  // loads issued together, added in order
+    cplx v[RED_AHEAD];
+#pragma unroll
+    for (int j = 0; j < RED_AHEAD; j++) v[j] = part[(size_t)(c + j) * NF + e];
+#pragma unroll
+    for (int j = 0; j < RED_AHEAD; j++) {
+      s.re += v[j].re;
+      s.im += v[j].im;
+    }
+  }
+  for (; c < c1; c++) {
     cplx v = part[(size_t)c * NF + e];
     s.re += v.re;
     s.im += v.im;
@@ -118,7 +132,19 @@ __global__ void __launch_bounds__(256) k_m2l_reduce32(uint32_t nc, const uint32_
   while ((n + 1) * (n + 2) / 2 <= e) n++;
   int m = e - n * (n + 1) / 2;
   double sr = 0.0, si = 0.0;
-  for (uint32_t c = choff[t]; c < choff[t + 1]; c++) {
+  const uint32_t c0 = choff[t], c1 = choff[t + 1];
+  uint32_t c = c0;
+  for (; c + RED_AHEAD <= c1; c += RED_AHEAD) {  // loads issued together, added in order
+    cplxf v[RED_AHEAD];
+#pragma unroll
+    for (int j = 0; j < RED_AHEAD; j++) v[j] = part[(size_t)(c + j) * NF + e];
+#pragma unroll
+    for (int j = 0; j < RED_AHEAD; j++) {
+      sr += (double)v[j].re;
+      si += (double)v[j].im;
+    }
+  }
+  for (; c < c1; c++) {
     const cplxf v = part[(size_t)c * NF + e];
     sr += (double)v.re;
     si += (double)v.im;

@@ -77,19 +77,22 @@ constexpr int NACC = 19;  // F00, F1 (3), F2 (5), L, G (3), H (6)
 
 __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* __restrict__ tptr,
                                                   const uint64_t* __restrict__ skeys, uint64_t smask,
-                                                  DevCells c, const double* __restrict__ q,
+                                                  const double4* __restrict__ cpm,
+                                                  const double* __restrict__ q,
                                                   double soft2, int D, cplx* __restrict__ locals,
                                                   double* __restrict__ lap) {
   const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   if (t >= nc) return;
-  const double tx = c.cx[t], ty = c.cy[t], tz = c.cz[t];
+  const double4 tc = cpm[t];
+  const double tx = tc.x, ty = tc.y, tz = tc.z;
   double acc[NACC];
 #pragma unroll
   for (int k = 0; k < NACC; k++) acc[k] = 0.0;
   for (uint32_t j = tptr[t] + lane; j < tptr[t + 1]; j += 64) {
     const uint32_t src = (uint32_t)(skeys[j] & smask);
-    const double rx = tx - c.cx[src], ry = ty - c.cy[src], rz = tz - c.cz[src];
+    const double4 sc = cpm[src];
+    const double rx = tx - sc.x, ry = ty - sc.y, rz = tz - sc.z;
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double s2 = r2 + soft2;
     const double is = 1.0 / sqrt(s2);
@@ -100,7 +103,7 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     const double hk = 105.0 * soft2 * is9;                             // Hess Lap K, r r^T part
     const double l4 = 15.0 * soft2 * (3.0 * is7 - 7.0 * r2 * is9);     // Lap^2 K
     const double g4 = 15.0 * soft2 * (63.0 * r2 * is11 - 35.0 * is9);  // grad Lap^2 K / r
-    const double ms = c.mass[src];
+    const double ms = sc.w;
     const double w = q[src] / 6.0;
     // source side: w Lap K, its gradient, the traceless part of its Hessian
     const double gw = w * gk, hw = 0.5 * w * hk;
@@ -170,11 +173,11 @@ void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, con
 }
 
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
-                      DevCells c, const double* q, double soft2, int p, cplx* locals, double* lap,
-                      hipStream_t s) {
+                      const double4* cpm, const double* q, double soft2, int p, cplx* locals,
+                      double* lap, hipStream_t s) {
   const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_m2l_trace, dim3((nc + 3) / 4), dim3(256), 0, s, nc, tptr, skeys, smask, c,
-                     q, soft2, hdim(p), locals, lap);
+  hipLaunchKernelGGL(k_m2l_trace, dim3((nc + 3) / 4), dim3(256), 0, s, nc, tptr, skeys, smask,
+                     cpm, q, soft2, hdim(p), locals, lap);
 }
 
 }  // namespace fmmh

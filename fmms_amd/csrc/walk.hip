@@ -23,7 +23,8 @@ struct __align__(64) WalkRec {
 };
 
 __global__ void __launch_bounds__(256) k_pack_walk(uint32_t nc, DevCells c,
-                                                  WalkRec* __restrict__ rec) {
+                                                  WalkRec* __restrict__ rec,
+                                                  double4* __restrict__ cpm) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nc) return;
   WalkRec r;
@@ -34,6 +35,7 @@ __global__ void __launch_bounds__(256) k_pack_walk(uint32_t nc, DevCells c,
 #pragma unroll
   for (int o = 0; o < 8; o++) r.ch[o] = c.children[(size_t)i * 8 + o];
   rec[i] = r;
+  cpm[i] = make_double4(r.cx, r.cy, r.cz, c.mass[i]);
 }
 
 size_t walk_rec_bytes() { return sizeof(WalkRec); }
@@ -51,8 +53,9 @@ void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStre
                      (uint2*)front);
 }
 
-void launch_pack_walk(uint32_t nc, DevCells c, void* rec, hipStream_t s) {
-  hipLaunchKernelGGL(k_pack_walk, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, (WalkRec*)rec);
+void launch_pack_walk(uint32_t nc, DevCells c, void* rec, double4* cpm, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_walk, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, (WalkRec*)rec,
+                     cpm);
 }
 
 // mac.rs:23-61 error estimate of cell a acting on cell b (r > 0 here).
