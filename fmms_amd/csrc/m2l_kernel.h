@@ -43,6 +43,16 @@ __device__ __forceinline__ void cmac(cplxf& f, float ar, float ai, float br, flo
   f.im = fmaf(ai, br, f.im);
 }
 
+// f += a * (br + i bi) for a real factor a.
+__device__ __forceinline__ void cmac_real(cplx& f, double a, double br, double bi) {
+  f.re = fma(a, br, f.re);
+  f.im = fma(a, bi, f.im);
+}
+__device__ __forceinline__ void cmac_real(cplxf& f, float a, float br, float bi) {
+  f.re = fmaf(a, br, f.re);
+  f.im = fmaf(a, bi, f.im);
+}
+
 __device__ __forceinline__ double m2l_rsqrt(double x) { return rsqrt(x); }
 __device__ __forceinline__ float m2l_rsqrt(float x) {
   const float y = rsqrtf(x);
@@ -77,10 +87,10 @@ __device__ __forceinline__ void m2l_accumulate(R x, R y, R z, R soft2, const C* 
         if (q + 2 <= s) {
           const R bq = (R)((s - 1) * (s - 1) - q * q) * ks;
           T[q].re = (a * T1[q].re - bq * T2[q].re) * ir2;
-          T[q].im = (a * T1[q].im - bq * T2[q].im) * ir2;
+          T[q].im = q == 0 ? R(0) : (a * T1[q].im - bq * T2[q].im) * ir2;  // Theta_s^0 is real
         } else if (q == s - 1) {
           T[q].re = a * T1[q].re * ir2;
-          T[q].im = a * T1[q].im * ir2;
+          T[q].im = q == 0 ? R(0) : a * T1[q].im * ir2;
         } else {  // q == s: (2s-1)(x+iy)/r^2 * Theta_{s-1}^{s-1}
           const R f = (R)(2 * s - 1) * ir2;
           const R wr = x * f, wi = y * f;
@@ -121,7 +131,16 @@ __device__ __forceinline__ void m2l_accumulate(R x, R y, R z, R soft2, const C* 
             br = sg * T[-q].re;
             bi = -sg * T[-q].im;
           }
-          cmac(F[hpos(n, m)], ar, ai, br, bi);
+          // Theta_s^0 is real (its imaginary part is an exact zero), and so is
+          // M_k^0 up to rounding residue in cells built by M2M: those terms
+          // take the real-factor product (2 FMAs instead of 4).
+          if (q == 0) {
+            cmac_real(F[hpos(n, m)], br, ar, ai);  // (ar + i ai) * br
+          } else if (l == 0) {
+            cmac_real(F[hpos(n, m)], ar, br, bi);  // ar * (br + i bi)
+          } else {
+            cmac(F[hpos(n, m)], ar, ai, br, bi);
+          }
         }
       }
     }
