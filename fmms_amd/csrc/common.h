@@ -51,6 +51,14 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t* total) 
   return x - v;
 }
 
+// Lane k's value (k wave-uniform) in every lane.
+__device__ __forceinline__ double lane_read(double v, uint32_t k) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)k);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
 // b % 8). Remapping gives each XCD a contiguous range of logical blocks, so
 // neighbouring work (which shares source data) shares one L2.

@@ -24,21 +24,32 @@ namespace fmmh {
 
 constexpr int ST = 256;
 
-// q of each leaf from its bodies (slot order).
+// q of each leaf from its bodies, one wave per leaf: the lanes form each
+// body's |x - z|^2 (64 bodies at a time, coalesced) and the sum runs in slot
+// order over the values read from the lanes that hold them.
 __global__ void __launch_bounds__(ST) k_trace_leaf(const uint32_t* __restrict__ leaf_ids,
                                                   uint32_t nleaves, DevCells c, DevBodies b,
                                                   double* __restrict__ q) {
-  const uint32_t i = blockIdx.x * ST + threadIdx.x;
+  const uint32_t i = blockIdx.x * (ST / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
   if (i >= nleaves) return;
   const uint32_t id = leaf_ids[i];
   const double zx = c.cx[id], zy = c.cy[id], zz = c.cz[id];
   double acc = 0.0;
-  const uint32_t f = c.first[id], e = f + c.count[id];
-  for (uint32_t k = f; k < e; k++) {
-    const double dx = b.x[k] - zx, dy = b.y[k] - zy, dz = b.z[k] - zz;
-    acc = fma(b.m[k], dx * dx + dy * dy + dz * dz, acc);
+  const uint32_t f = c.first[id], n = c.count[id];
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t k = f + base + lane;
+    const bool have = base + lane < n;
+    double mk = 0.0, d2 = 0.0;
+    if (have) {
+      const double dx = b.x[k] - zx, dy = b.y[k] - zy, dz = b.z[k] - zz;
+      mk = b.m[k];
+      d2 = dx * dx + dy * dy + dz * dz;
+    }
+    const uint32_t cnt = min(64u, n - base);
+    for (uint32_t j = 0; j < cnt; j++) acc = fma(lane_read(mk, j), lane_read(d2, j), acc);
   }
-  q[id] = acc;
+  if (lane == 0) q[id] = acc;
 }
 
 // Parallel-axis upward pass: q_P = sum_c q_c + 2 D_c.(z_c - z_P) + M_c |z_c - z_P|^2,
@@ -161,8 +172,8 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
 void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
                        double* q, hipStream_t s) {
   if (nleaves == 0) return;
-  hipLaunchKernelGGL(k_trace_leaf, dim3((nleaves + ST - 1) / ST), dim3(ST), 0, s, leaf_ids,
-                     nleaves, c, b, q);
+  hipLaunchKernelGGL(k_trace_leaf, dim3((nleaves + ST / 64 - 1) / (ST / 64)), dim3(ST), 0, s,
+                     leaf_ids, nleaves, c, b, q);
 }
 
 void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, const cplx* mom,

@@ -627,14 +627,6 @@ __global__ void __launch_bounds__(TB) k_geometry(const uint32_t* __restrict__ ce
 // test the bodies after it again, which is exactly the sequential pass.
 constexpr int GEO_WAVES = 4;
 
-// Lane k's value (k wave-uniform) in every lane.
-__device__ __forceinline__ double lane_read(double v, uint32_t k) {
-  const uint64_t u = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, (int)k);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), (int)k);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 __device__ __forceinline__ double wave_max_pos(double v) {  // v >= +0 in every lane
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

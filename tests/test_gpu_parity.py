@@ -62,6 +62,12 @@ TREE_CASES = {
     "offset_xmin_quirk": (offset_cloud, 16, 0),
     "sphere_plummer2048": (lambda: O.plummer(2048, seed=13), 16, 1),
     "sphere_grid64": (grid64, 1, 1),
+    # leaves of more than 64 bodies: the leaf geometry runs over several
+    # 64-body chunks (ordered sums, grow passes across chunks)
+    "plummer4096_leaf200": (lambda: O.plummer(4096, seed=99), 200, 0),
+    "sphere_plummer4096_leaf200": (lambda: O.plummer(4096, seed=99), 200, 1),
+    "sphere_dup_ties_leaf40": (dup_cloud, 40, 1),
+    "sphere_coincident130": (lambda: (np.full((130, 3), 0.25), np.linspace(1.0, 2.0, 130)), 2, 1),
 }
 
 
