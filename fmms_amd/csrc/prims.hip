@@ -1,8 +1,8 @@
 // prims.hip — HBM-bound primitives for gfx950: device-wide exclusive scan
 // (u32) and a stable LSD radix sort of (u64 key, u32 value) pairs.
 //
-// Radix sort: 8-bit digits, tiles of 4096 elements per 256-thread block
-// (4 waves x 16 rounds x 64 lanes). Per pass: a per-tile digit histogram,
+// Radix sort: 8-bit digits, tiles of 2048 elements per 256-thread block
+// (4 waves x 8 rounds x 64 lanes). Per pass: a per-tile digit histogram,
 // an exclusive scan of the digit-major histogram, then a stable scatter in
 // which each wave ranks its elements with an 8-ballot match (lanes with the
 // same digit) and per-wave LDS counters. Elements keep their relative order
@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_tiles(const uint32_t* __restric
     size_t e = base + (size_t)j * SCAN_T + threadIdx.x;
     if (e < n) out[e] = tile[j * SCAN_T + threadIdx.x];
   }
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+  // one tile: its total is the grand total (tile_sums[ntiles = 1])
+  if (threadIdx.x == 0) tile_sums[gridDim.x == 1 ? 1 : blockIdx.x] = total;
 }
 
 // One block scans the tile sums in place; tile_sums[ntiles] = grand total.
@@ -95,6 +96,26 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_add(uint32_t* __restrict__ out,
   }
 }
 
+// Up to SCAN_T tiles: each block adds the totals of the tiles before it
+// (one per thread, then a block sum), so no separate pass scans the tile
+// totals; the last block stores the grand total in sums[ntiles].
+__global__ void __launch_bounds__(SCAN_T) k_scan_add_small(uint32_t* __restrict__ out, size_t n,
+                                                          uint32_t* __restrict__ sums,
+                                                          uint32_t ntiles) {
+  __shared__ uint32_t sh[SCAN_T / 64];
+  const uint32_t v = threadIdx.x < blockIdx.x ? sums[threadIdx.x] : 0u;
+  uint32_t add;
+  block_excl_scan(v, &add, sh);
+  if (blockIdx.x == ntiles - 1 && threadIdx.x == 0) sums[ntiles] = add + sums[ntiles - 1];
+  if (add == 0) return;
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int j = 0; j < SCAN_I; j++) {
+    size_t e = base + (size_t)j * SCAN_T + threadIdx.x;
+    if (e < n) out[e] += add;
+  }
+}
+
 size_t scan_tmp_words(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 // Exclusive scan; in may equal out. tmp needs scan_tmp_words(n) words; the
@@ -106,8 +127,13 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, s, tmp, (size_t)0);
   } else {
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_T), 0, s, in, out, n, tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, s, tmp, ntiles);
-    if (ntiles > 1) hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_T), 0, s, out, n, tmp);
+    if (ntiles > (size_t)SCAN_T) {
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_T), 0, s, tmp, ntiles);
+      hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(SCAN_T), 0, s, out, n, tmp);
+    } else if (ntiles > 1) {
+      hipLaunchKernelGGL(k_scan_add_small, dim3(ntiles), dim3(SCAN_T), 0, s, out, n, tmp,
+                         (uint32_t)ntiles);
+    }
   }
   if (total_dev) *total_dev = tmp + ntiles;
 }
