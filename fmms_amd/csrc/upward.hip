@@ -55,9 +55,10 @@ __device__ __forceinline__ void p2m_column(uint32_t id, DevCells c, DevBodies b,
   for (int n = 0; n <= P; n++) acc[n] = accn[n] = cz();
   const uint32_t f = c.first[id], e = f + c.count[id];
   for (uint32_t k = f; k < e; k++) {
-    double x = b.x[k] - zx, y = b.y[k] - zy, z = b.z[k] - zz;
+    const double4 q = b.xyzm[k];  // one 32-byte load (a lane's leaf differs from its neighbours')
+    double x = q.x - zx, y = q.y - zy, z = q.z - zz;
     double r2 = x * x + y * y + z * z;
-    upsilon_column_acc<P>(x, y, z, r2, M, b.m[k], acc, accn);
+    upsilon_column_acc<P>(x, y, z, r2, M, q.w, acc, accn);
   }
   cplx* out = moments + (size_t)id * D;
 #pragma unroll
