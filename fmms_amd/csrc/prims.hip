@@ -152,9 +152,12 @@ __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint64_t* __restrict__ k
                                                  uint32_t nblk) {
   __shared__ uint32_t h[4][256];
   const int w = threadIdx.x >> 6;
+  // neighbouring tiles share the cache lines of each digit row of the
+  // histogram: keep them on one XCD's L2
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < 4 * 256; i += RS_T) (&h[0][0])[i] = 0;
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * RS_TILE;
+  const size_t base = (size_t)tile * RS_TILE;
 #pragma unroll 4
   for (int j = 0; j < RS_I; j++) {
     size_t e = base + (size_t)j * RS_T + threadIdx.x;
@@ -162,7 +165,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint64_t* __restrict__ k
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 256; i += RS_T)
-    hist[(size_t)i * nblk + blockIdx.x] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+    hist[(size_t)i * nblk + tile] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
 }
 
 // Scatter of one 8-bit pass. Each wave ranks its keys per digit (ballot
@@ -183,9 +186,10 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict_
   __shared__ uint64_t sk[RS_TILE];
   __shared__ uint32_t sv[VALS ? RS_TILE : 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);  // as in k_rs_hist
   for (int i = threadIdx.x; i < 4 * 256; i += RS_T) (&wcnt[0][0])[i] = 0;
   __syncthreads();
-  const size_t bbase = (size_t)blockIdx.x * RS_TILE;
+  const size_t bbase = (size_t)tile * RS_TILE;
   const size_t wbase = bbase + (size_t)w * RS_WAVE_SPAN;
   const uint64_t lt = lanemask_lt();
   uint64_t kk[RS_I];
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint64_t* __restrict_
   for (uint32_t i = threadIdx.x; i < cnt_blk; i += RS_T) {
     const uint64_t key = sk[i];
     const uint32_t d = (uint32_t)(key >> shift) & 255u;
-    const size_t pos = (size_t)offs[(size_t)d * nblk + blockIdx.x] + (i - dstart[d]);
+    const size_t pos = (size_t)offs[(size_t)d * nblk + tile] + (i - dstart[d]);
     kout[pos] = key;
     if (VALS) vout[pos] = sv[i];
   }
