@@ -92,6 +92,7 @@ SIGNATURES = {
     "fmmh_evaluate_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
     "fmmh_accel_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
     "fmmh_num_cells": (_sz, [_vp]),
+    "fmmh_device_bytes": (_sz, [_vp]),
     "fmmh_get_tree": (ct.c_int, [_vp, _pu32, _pu64, _pu32, _pu32, _pi32, _pu8, _pu32, _pd, _pd, _pd]),
     "fmmh_get_list_sizes": (ct.c_int, [_vp, ct.POINTER(_sz), ct.POINTER(_sz)]),
     "fmmh_get_lists": (ct.c_int, [_vp, _pu32, _pu32]),
@@ -224,6 +225,10 @@ class Fmm:
     @property
     def n_cells(self):
         return int(lib().fmmh_num_cells(self._h))
+
+    def device_bytes(self):
+        """Device memory this context holds (bytes)."""
+        return int(lib().fmmh_device_bytes(self._h))
 
     def run(self, pos, mass):
         mass = _f64(mass)

@@ -261,12 +261,13 @@ struct fmmh_ctx {
     ensure_scan(n);
     mark(ST_TREE);
     HCHECK(hipMemsetAsync(flags.p, 0, 64, s));
+    uint64_t* off = nullptr;
     if (nsys == 1) {
       launch_bbox(pos, n, (double*)bbox_partial.p, (BBox*)bbox.p, s);
       launch_keys(pos, mass, n, (const BBox*)bbox.p, (uint64_t*)keys.p, (uint32_t*)vals.p,
                   (uint32_t*)flags.p, s);
     } else {
-      uint64_t* off = ensure<uint64_t>(d_sysoff, (nsys + 1) * 8);
+      off = ensure<uint64_t>(d_sysoff, (nsys + 1) * 8);
       uint64_t* h = (uint64_t*)host_stage((nsys + 1) * 8);
       std::memcpy(h, sys_off.data(), (nsys + 1) * 8);
       HCHECK(hipMemcpyAsync(off, h, (nsys + 1) * 8, hipMemcpyHostToDevice, s));
@@ -278,6 +279,19 @@ struct fmmh_ctx {
                               (uint32_t*)vals_alt.p, n, 0, 63, (uint32_t*)sort_tmp.p, s);
     keys_sorted = (uint64_t*)(flip ? keys_alt.p : keys.p);
     order = (uint32_t*)(flip ? vals_alt.p : vals.p);
+    if (nsys > 1) {
+      // bodies now in standalone-key order: take the system prefix, then a
+      // stable sort on the prefix bits gives (system, standalone key) order
+      launch_keys_prefix(keys_sorted, order, n, off, nsys, vlev, s);
+      uint64_t* k2 = (uint64_t*)(flip ? keys.p : keys_alt.p);
+      uint32_t* v2 = (uint32_t*)(flip ? vals.p : vals_alt.p);
+      const int f2 = radix_sort_u64(keys_sorted, order, k2, v2, n, 63 - 3 * vlev, 63,
+                                    (uint32_t*)sort_tmp.p, s);
+      if (f2) {
+        keys_sorted = k2;
+        order = v2;
+      }
+    }
     launch_gather(pos, mass, order, n, bodies(), s);
     // level-by-level octant splits
     HCHECK(hipMemsetAsync(chain.p, 0, n * 4, s));
@@ -1125,6 +1139,8 @@ int fmmh_evaluate_frozen(fmmh_ctx* ctx, size_t n, const double* pos, const doubl
 }
 
 size_t fmmh_num_cells(const fmmh_ctx* ctx) { return ctx && ctx->built ? ctx->nc : 0; }
+
+size_t fmmh_device_bytes(const fmmh_ctx* ctx) { return ctx ? ctx->used : 0; }
 
 int fmmh_get_tree(fmmh_ctx* ctx, uint32_t* order, uint64_t* keys_sorted, uint32_t* first,
                   uint32_t* count, int32_t* children, uint8_t* leaf, uint32_t* level, double* cen,

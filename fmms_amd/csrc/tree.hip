@@ -182,10 +182,22 @@ __global__ void __launch_bounds__(TB) k_keys(const double* __restrict__ pos,
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
 }
 
-// Ensemble keys: the body's Morton key within its own system's box (the
-// standalone key) with its low 3V bits dropped, under a 3V-bit system prefix
-// (V virtual levels, nsys <= 8^V). Below level V the forest splits exactly
-// like each system's standalone tree.
+__device__ __forceinline__ uint32_t system_of(const uint64_t* __restrict__ off, uint32_t nsys,
+                                              uint64_t i) {
+  uint32_t lo = 0, hi = nsys;  // off[lo] <= i < off[lo + 1]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Ensemble keys, first half: the body's Morton key within its own system's
+// box (its standalone key). The bodies are sorted by these keys first, so
+// bodies whose keys agree above the low 3V bits keep their standalone order.
 __global__ void __launch_bounds__(TB) k_keys_sys(const double* __restrict__ pos,
                                                 const double* __restrict__ mass, size_t n,
                                                 const uint64_t* __restrict__ off, uint32_t nsys,
@@ -195,23 +207,36 @@ __global__ void __launch_bounds__(TB) k_keys_sys(const double* __restrict__ pos,
                                                 uint32_t* __restrict__ flags) {
   size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
   if (i >= n) return;
-  uint32_t lo = 0, hi = nsys;  // system s with off[s] <= i < off[s+1]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= i)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const BBox* bb = bbs + lo;
+  const BBox* bb = bbs + system_of(off, nsys, i);
   const double lo0 = bb->lo[0], inv = bb->inv_side;
   double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2], m = mass[i];
   uint64_t key = dilate21(quantise(z, lo0, inv));
   key |= dilate21(quantise(y, lo0, inv)) << 1;
   key |= dilate21(quantise(x, lo0, inv)) << 2;
-  keys[i] = ((uint64_t)lo << (63 - 3 * vlev)) | (key >> (3 * vlev));
+  keys[i] = key;
   vals[i] = (uint32_t)i;
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
+}
+
+// Ensemble keys, second half (on the bodies sorted by standalone key): the
+// standalone key with its low 3V bits dropped, under a 3V-bit system prefix
+// (V virtual levels, nsys <= 8^V). A stable sort on the prefix bits then
+// orders the bodies by (system, standalone key), and below level V the
+// forest splits exactly like each system's standalone tree.
+__global__ void __launch_bounds__(TB) k_keys_prefix(uint64_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ ids, size_t n,
+                                                   const uint64_t* __restrict__ off, uint32_t nsys,
+                                                   int vlev) {
+  size_t k = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t sys = system_of(off, nsys, ids[k]);
+  keys[k] = ((uint64_t)sys << (63 - 3 * vlev)) | (keys[k] >> (3 * vlev));
+}
+
+void launch_keys_prefix(uint64_t* keys, const uint32_t* ids, size_t n, const uint64_t* off,
+                        uint32_t nsys, int vlev, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys_prefix, dim3((n + TB - 1) / TB), dim3(TB), 0, s, keys, ids, n, off,
+                     nsys, vlev);
 }
 
 void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,

@@ -21,12 +21,25 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Fmm, FmmError, hip, ic_disk, ic_hernquist, ic_plummer
+from . import FMMH_E_OVER_BUDGET, Fmm, FmmError, hip, ic_disk, ic_hernquist, ic_plummer
 
 # Device bytes per body of one evaluation context at p = 8 (bodies, keys,
 # cells, expansions and lists), measured on the 1M-body Plummer workload and
-# rounded up; used for admission under the HBM budget.
+# rounded up: the starting estimate for admission under the HBM budget.
 BYTES_PER_BODY = 1600
+# Packed state per body: pos, vel, acc (3 doubles each), mass, pot.
+STATE_BYTES = 88
+# Allowance for what the server's two contexts hold whatever the batch
+# (size-independent buffers, growth slack of the arenas, the admission
+# context's buffers for its largest newcomer so far).
+OVERHEAD_BYTES = 256 << 20
+
+
+def bytes_per_body(p):
+    """Starting estimate of one context's device bytes per body at expansion
+    order p: a fixed part (bodies, keys, cells, lists) and the expansions,
+    which grow with (p + 1)^2 (BYTES_PER_BODY at p = 8)."""
+    return 600.0 + 1000.0 * (p + 1) ** 2 / 81.0
 
 
 @dataclass
@@ -42,6 +55,7 @@ class Request:
     admitted: float = math.nan
     finished: float = math.nan
     steps_done: int = 0
+    rejected: bool = False  # larger than the whole HBM budget: never admitted
     final: tuple = field(default=None, repr=False)
 
     @property
@@ -83,8 +97,12 @@ class Server:
                        precision=precision, soft_mac=soft_mac)
         self.dt = dt
         self.hbm_budget = float(hbm_budget)
-        self.ctx = Fmm(None, None, **self.kw)
-        self.admit_ctx = Fmm(None, None, **self.kw)
+        # each context also enforces the budget on its own allocations
+        self.ctx = Fmm(None, None, hbm_budget_bytes=self.hbm_budget, **self.kw)
+        self.admit_ctx = Fmm(None, None, hbm_budget_bytes=self.hbm_budget, **self.kw)
+        # device bytes per body: the estimate for p, raised to what the
+        # ensemble context actually holds once it has run
+        self.bpb = bytes_per_body(p)
         self.active = []      # admitted systems, in packing order
         self.queue = []       # arrived, not yet admitted
         self.packed = None    # dict name -> DeviceArray over all active systems
@@ -94,17 +112,30 @@ class Server:
 
     # ---------------------------------------------------------- admission
     def _bytes(self, n):
-        return BYTES_PER_BODY * n
+        return (self.bpb + STATE_BYTES) * n
 
     def _in_use(self):
-        return sum(self._bytes(s.req.n) for s in self.active)
+        return OVERHEAD_BYTES + sum(self._bytes(s.req.n) for s in self.active)
+
+    def _fits_alone(self, n):
+        return OVERHEAD_BYTES + self._bytes(n) <= self.hbm_budget
 
     def submit(self, req):
+        """Queue a request; one that could not fit even alone is rejected
+        (FmmError with code FMMH_E_OVER_BUDGET) instead of waiting forever."""
+        if not self._fits_alone(req.n):
+            req.rejected = True
+            raise FmmError(FMMH_E_OVER_BUDGET,
+                           f"request {req.rid}: {req.n} bodies need about "
+                           f"{self._bytes(req.n):.3g} bytes, over the {self.hbm_budget:.3g}-byte budget")
         self.queue.append(req)
 
     def _admit(self, now):
         kept = []
         for req in self.queue:
+            if not self._fits_alone(req.n):  # the measured footprint outgrew the budget
+                req.rejected = True
+                continue
             if self._in_use() + self._bytes(req.n) > self.hbm_budget:
                 kept.append(req)
                 continue
@@ -185,6 +216,8 @@ class Server:
         p = self.packed
         self.ctx.leapfrog_device(total, p["pos"].ptr, p["vel"].ptr, p["mass"].ptr, p["acc"].ptr,
                                  p["pot"].ptr, self.dt, 1, refresh_every=1, acc_valid=True)
+        if total >= 100_000:  # size-independent buffers no longer dominate the arena
+            self.bpb = max(self.bpb, self.ctx.device_bytes() / total)
         done = []
         for s in self.active:
             s.req.steps_done += 1
@@ -212,7 +245,10 @@ class Server:
             if max_seconds is not None and now > max_seconds:
                 break
             while i < len(pending) and pending[i].arrival <= now:
-                self.submit(pending[i])
+                try:
+                    self.submit(pending[i])
+                except FmmError:
+                    pass  # marked rejected; it never runs
                 i += 1
             if not self.queue and not self.active:
                 time.sleep(max(0.0, min(pending[i].arrival - now, 0.01)))
@@ -231,12 +267,13 @@ class Server:
                 "mean_batch_bodies": float(bodies.mean())}
 
 
-def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4, soft_mac=0.0):
+def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4, soft_mac=0.0,
+              precision="fp64"):
     """The same system stepped on its own (the reference behaviour: one Sim)."""
     from . import Integrator
 
     f = Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=softening,
-            soft_mac=soft_mac)
+            soft_mac=soft_mac, precision=precision)
     it = Integrator(f, req.pos, req.vel, req.mass)
     it.step(dt, steps=0)
     t0 = time.perf_counter()
@@ -246,4 +283,4 @@ def run_alone(req, p=8, theta=0.5, softening=1e-3, n_leaf=32, dt=1e-4, soft_mac=
     return it.state(), el
 
 
-__all__ = ["Request", "Server", "make_request", "run_alone", "FmmError"]
+__all__ = ["OVERHEAD_BYTES", "Request", "STATE_BYTES", "Server", "bytes_per_body", "make_request", "run_alone", "FmmError"]

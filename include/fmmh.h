@@ -49,8 +49,11 @@ extern "C" {
 /* Precision forms */
 #define FMMH_PREC_FP64 0      /* plain FP64 everywhere (the drop-in) */
 #define FMMH_PREC_FP32BULK 1  /* mixed precision: P2P pair terms in FP32 on positions
-                                 relative to the target leaf's centre, FP64 tile and
-                                 final accumulation; everything else FP64 */
+                                 relative to the target leaf's centre and M2L in FP32
+                                 on radius-scaled operands (per-cell scale, scaled
+                                 moments, dimensionless offsets), with FP64 tile and
+                                 chunk sums and FP64 final accumulation; the tree,
+                                 the upward pass, L2L and L2P stay FP64 */
 
 /* Flags */
 #define FMMH_FLAG_TIMING 1u   /* record per-stage device times (hipEvents) */
@@ -98,6 +101,12 @@ int fmmh_evaluate_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const dou
                          double* d_accel, double* d_pot);
 int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
                       double* d_accel, double* d_pot);
+
+/* Device bytes this context holds (its arena of stage buffers); callers that
+ * admit work under a memory budget read it after a call. No reference
+ * counterpart (the reference's GPU app sizes buffers up front,
+ * crates/visual/src/gpufmm.rs:1656-1691). */
+size_t fmmh_device_bytes(const fmmh_ctx* ctx);
 
 /* Tree of the last build (tree.rs:49-70), DFS-preorder cell ids. Any
  * output pointer may be NULL. children [nc][8], -1 where absent; cen [nc][3]. */

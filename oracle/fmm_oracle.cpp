@@ -12,12 +12,25 @@
 #include "fmm_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <thread>
 #include <vector>
+
+// ORC_TIMING: the CPU-baseline build (oracle/Makefile `timing`), compiled
+// for the host's own ISA with multiply-adds fused. It times the reference's
+// algorithm the way the reference runs it: 8-wide FP64 P2P with an rsqrt
+// estimate refined by two Newton steps (p2p.rs:117-326 takes that route on
+// AVX-512 hosts) and threads that take work dynamically (walk.rs:342-488
+// steals work). Its results differ from the parity build in the last bits;
+// it is never the checker.
+#if defined(ORC_TIMING) && defined(__AVX512F__)
+#include <immintrin.h>
+#define ORC_P2P_AVX512 1
+#endif
 
 namespace {
 
@@ -287,6 +300,58 @@ struct Bodies {  // bodies.rs:13-22, slot order
 // ----------------------------------------------------------------- p2p.rs
 // Scalar mutual kernels (p2p.rs:13-88, the "exact reference" path). Softening
 // (extension): r2 += soft2 when soft2 > 0, so soft2 == 0 is bit-identical.
+#ifdef ORC_P2P_AVX512
+// Timing build: body i against bodies [j0, j1) in 8-wide blocks, both sides
+// updated (the mutual kernel); the tail block is masked.
+inline void p2p_row_avx512(Bodies& b, size_t i, size_t j0, size_t j1, double soft2) {
+  const __m512d xi = _mm512_set1_pd(b.x[i]), yi = _mm512_set1_pd(b.y[i]),
+                zi = _mm512_set1_pd(b.z[i]), mi = _mm512_set1_pd(b.m[i]);
+  const __m512d e2 = _mm512_set1_pd(soft2), half = _mm512_set1_pd(0.5),
+                onehalf = _mm512_set1_pd(1.5);
+  __m512d ax = _mm512_setzero_pd(), ay = ax, az = ax, ph = ax;
+  for (size_t j = j0; j < j1; j += 8) {
+    const size_t left = j1 - j;
+    const __mmask8 k = left >= 8 ? (__mmask8)0xFF : (__mmask8)((1u << left) - 1u);
+    const __m512d xj = _mm512_maskz_loadu_pd(k, &b.x[j]);
+    const __m512d yj = _mm512_maskz_loadu_pd(k, &b.y[j]);
+    const __m512d zj = _mm512_maskz_loadu_pd(k, &b.z[j]);
+    const __m512d mj = _mm512_maskz_loadu_pd(k, &b.m[j]);
+    const __m512d dx = _mm512_sub_pd(xj, xi), dy = _mm512_sub_pd(yj, yi),
+                  dz = _mm512_sub_pd(zj, zi);
+    __m512d r2 = _mm512_fmadd_pd(dx, dx, _mm512_fmadd_pd(dy, dy, _mm512_fmadd_pd(dz, dz, e2)));
+    r2 = _mm512_mask_blend_pd(k, _mm512_set1_pd(1.0), r2);  // masked lanes: r = 1, m = 0
+    __m512d inv = _mm512_rsqrt14_pd(r2);
+    for (int it = 0; it < 2; it++) {  // Newton: y (3/2 - r2 y^2 / 2)
+      const __m512d hy2 = _mm512_mul_pd(_mm512_mul_pd(half, r2), _mm512_mul_pd(inv, inv));
+      inv = _mm512_mul_pd(inv, _mm512_sub_pd(onehalf, hy2));
+    }
+    const __m512d inv3 = _mm512_mul_pd(_mm512_mul_pd(inv, inv), inv);
+    const __m512d fj = _mm512_mul_pd(mj, inv3);
+    ax = _mm512_fmadd_pd(fj, dx, ax);
+    ay = _mm512_fmadd_pd(fj, dy, ay);
+    az = _mm512_fmadd_pd(fj, dz, az);
+    ph = _mm512_fmadd_pd(mj, inv, ph);
+    const __m512d fi = _mm512_mul_pd(mi, inv3);
+    _mm512_mask_storeu_pd(&b.ax[j], k, _mm512_fnmadd_pd(fi, dx, _mm512_maskz_loadu_pd(k, &b.ax[j])));
+    _mm512_mask_storeu_pd(&b.ay[j], k, _mm512_fnmadd_pd(fi, dy, _mm512_maskz_loadu_pd(k, &b.ay[j])));
+    _mm512_mask_storeu_pd(&b.az[j], k, _mm512_fnmadd_pd(fi, dz, _mm512_maskz_loadu_pd(k, &b.az[j])));
+    _mm512_mask_storeu_pd(&b.pot[j], k,
+                          _mm512_fmadd_pd(mi, inv, _mm512_maskz_loadu_pd(k, &b.pot[j])));
+  }
+  b.ax[i] += _mm512_reduce_add_pd(ax);
+  b.ay[i] += _mm512_reduce_add_pd(ay);
+  b.az[i] += _mm512_reduce_add_pd(az);
+  b.pot[i] += _mm512_reduce_add_pd(ph);
+}
+
+void p2p_self(Bodies& b, size_t first, size_t count, double soft2) {
+  for (size_t i = 0; i + 1 < count; i++) p2p_row_avx512(b, first + i, first + i + 1, first + count, soft2);
+}
+
+void p2p_pair(Bodies& b, size_t af, size_t ac, size_t bf, size_t bc, double soft2) {
+  for (size_t i = 0; i < ac; i++) p2p_row_avx512(b, af + i, bf, bf + bc, soft2);
+}
+#else
 void p2p_self(Bodies& b, size_t first, size_t count, double soft2) {
   for (size_t i = 0; i < count; i++) {
     double xi = b.x[first + i], yi = b.y[first + i], zi = b.z[first + i], mi = b.m[first + i];
@@ -346,6 +411,7 @@ void p2p_pair(Bodies& b, size_t af, size_t ac, size_t bf, size_t bc, double soft
     b.pot[af + i] += poti;
   }
 }
+#endif  // ORC_P2P_AVX512
 
 // -------------------------------------------------------------- translate.rs
 // translate.rs:47-136 mutual naive M2L on explicit centres (z = s = cen).
@@ -687,6 +753,34 @@ void parallel_for(int nthreads, size_t n, F fn) {  // contiguous chunks [lo, hi)
     ts.emplace_back([=, &fn] { fn(t, lo, hi); });
   }
   for (auto& th : ts) th.join();
+}
+
+// Loops whose iterations differ in cost. The timing build hands out small
+// chunks from a shared counter (a thread may get several chunks, so fn must
+// not assume one call per thread); the parity build keeps the contiguous
+// split of parallel_for, so its results do not depend on timing.
+template <class F>
+void parallel_for_dyn(int nthreads, size_t n, F fn) {
+#ifdef ORC_TIMING
+  if (nthreads <= 1 || n < 2) {
+    fn(0, (size_t)0, n);
+    return;
+  }
+  const size_t chunk = std::max<size_t>(1, n / ((size_t)nthreads * 32));
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> ts;
+  for (int t = 0; t < nthreads; t++)
+    ts.emplace_back([&, t] {
+      for (;;) {
+        const size_t lo = next.fetch_add(chunk);
+        if (lo >= n) break;
+        fn(t, lo, std::min(n, lo + chunk));
+      }
+    });
+  for (auto& th : ts) th.join();
+#else
+  parallel_for(nthreads, n, fn);
+#endif
 }
 
 }  // namespace
@@ -1434,7 +1528,7 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
   if (use_err) {
     t->pows = multipole_power(t->moments.data(), nc, p);
     t->f.assign(n, 0.0);
-    parallel_for(nthreads, n, [&](int, size_t lo, size_t hi) {
+    parallel_for_dyn(nthreads, n, [&](int, size_t lo, size_t hi) {
       std::vector<size_t> stack;
       stack.reserve(256);
       for (size_t i = lo; i < hi; i++) t->f[i] = f_walk(*t, i, stack);
@@ -1517,13 +1611,15 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
     }
     std::vector<std::vector<uint32_t>> tm2l(nthreads), tp2p(nthreads);
     std::vector<Bodies> tb(nthreads);
-    parallel_for(nthreads, seed.size(), [&](int th, size_t lo, size_t hi) {
+    parallel_for_dyn(nthreads, seed.size(), [&](int th, size_t lo, size_t hi) {
       Bodies& my = tb[th];
-      my = B;
-      std::fill(my.ax.begin(), my.ax.end(), 0.0);
-      std::fill(my.ay.begin(), my.ay.end(), 0.0);
-      std::fill(my.az.begin(), my.az.end(), 0.0);
-      std::fill(my.pot.begin(), my.pot.end(), 0.0);
+      if (my.x.empty()) {  // first chunk of this thread: private accumulators
+        my = B;
+        std::fill(my.ax.begin(), my.ax.end(), 0.0);
+        std::fill(my.ay.begin(), my.ay.end(), 0.0);
+        std::fill(my.az.begin(), my.az.end(), 0.0);
+        std::fill(my.pot.begin(), my.pot.end(), 0.0);
+      }
       std::vector<std::pair<size_t, size_t>> st;
       for (size_t s = lo; s < hi; s++) {
         st.push_back(seed[s]);
@@ -1627,7 +1723,7 @@ int orc_run(orc_ctx* t, const double* pos, const double* mass, int p, double eps
   // l2p_scatter (translate.rs:312-407)
   std::memset(accel, 0, 3 * n * sizeof(double));
   std::memset(pot, 0, n * sizeof(double));
-  parallel_for(nthreads, nc, [&](int, size_t lo, size_t hi) {
+  parallel_for_dyn(nthreads, nc, [&](int, size_t lo, size_t hi) {
     std::vector<C> u(d);
     for (size_t c = lo; c < hi; c++) {
       const Cell& cell = t->cells[c];

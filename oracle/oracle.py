@@ -31,10 +31,22 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
 
 
-def _load():
-    if not os.path.exists(_LIB):
-        build()
-    lib = ct.CDLL(_LIB)
+_LIB_TIMING = os.path.join(_HERE, "liboracle_timing.so")
+
+
+def build_timing():
+    """Compile liboracle_timing.so (the CPU-baseline build: host ISA,
+    contraction on, 8-wide P2P, dynamic threads) on this machine."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "timing"], check=True)
+
+
+def _load(path=None):
+    path = path or _LIB
+    if not os.path.exists(path):
+        build_timing() if path == _LIB_TIMING else build()
+    lib = ct.CDLL(path)
     sig = {
         "orc_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
         "orc_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
@@ -98,6 +110,7 @@ def _load():
 
 
 _lib = None
+_lib_timing = None
 
 
 def lib():
@@ -105,6 +118,17 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+def timing_lib(rebuild=False):
+    """The timing build (never the checker). rebuild=True compiles it for
+    this host first, since -march=native code may not run on another CPU."""
+    global _lib_timing
+    if _lib_timing is None:
+        if rebuild:
+            build_timing()
+        _lib_timing = _load(_LIB_TIMING)
+    return _lib_timing
 
 
 def _p(a, ctype=_pd):
@@ -371,25 +395,28 @@ class OracleFmm:
     `refresh` rebuilds, `run` evaluates and keeps every intermediate."""
 
     def __init__(self, pos, mass, p=8, eps=1e-5, n_leaf=16, center_mode=0, theta_max=1.0,
-                 softening=0.0, soft_mac=0.0):
+                 softening=0.0, soft_mac=0.0, timing_build=False):
+        """timing_build=True runs the CPU-baseline build (timing_lib())
+        instead of the parity build."""
         self.p, self.eps, self.n_leaf = p, eps, n_leaf
         self.center_mode, self.theta_max, self.softening = center_mode, theta_max, softening
-        self._ctx = lib().orc_new()
-        lib().orc_set_soft_mac(self._ctx, soft_mac)
+        self._lib = timing_lib() if timing_build else lib()
+        self._ctx = self._lib.orc_new()
+        self._lib.orc_set_soft_mac(self._ctx, soft_mac)
         self.refresh(pos, mass)
 
     def __del__(self):
         if getattr(self, "_ctx", None):
-            lib().orc_free(self._ctx)
+            self._lib.orc_free(self._ctx)
             self._ctx = None
 
     def refresh(self, pos, mass):
         pos, mass = _f64(pos), _f64(mass)
-        rc = lib().orc_build(self._ctx, mass.shape[0], _p(pos), _p(mass), self.n_leaf, self.center_mode)
+        rc = self._lib.orc_build(self._ctx, mass.shape[0], _p(pos), _p(mass), self.n_leaf, self.center_mode)
         if rc != 0:
             raise ValueError("empty input")
         self.n = mass.shape[0]
-        self.ncells = lib().orc_ncells(self._ctx)
+        self.ncells = self._lib.orc_ncells(self._ctx)
 
     def tree(self):
         n, nc = self.n, self.ncells
@@ -404,11 +431,11 @@ class OracleFmm:
         t.cen = np.empty((nc, 3))
         t.mass = np.empty(nc)
         t.rho = np.empty(nc)
-        lib().orc_get_tree(self._ctx, _p(t.order, _pu32), _p(t.keys, _pu64), _p(t.first, _pu32),
+        self._lib.orc_get_tree(self._ctx, _p(t.order, _pu32), _p(t.keys, _pu64), _p(t.first, _pu32),
                            _p(t.count, _pu32), _p(t.children, _pi32), _p(t.leaf, _pu8),
                            _p(t.level, _pu32), _p(t.cen), _p(t.mass), _p(t.rho))
         lo, hi, inv = np.empty(3), np.empty(3), np.empty(1)
-        lib().orc_get_bbox(self._ctx, _p(lo), _p(hi), _p(inv))
+        self._lib.orc_get_bbox(self._ctx, _p(lo), _p(hi), _p(inv))
         t.lo, t.hi, t.inv_side = lo, hi, inv[0]
         return t
 
@@ -417,7 +444,7 @@ class OracleFmm:
         n = mass.shape[0]
         acc = np.empty((n, 3))
         pot = np.empty(n)
-        rc = lib().orc_run(self._ctx, _p(pos), _p(mass), self.p, self.eps, self.theta_max,
+        rc = self._lib.orc_run(self._ctx, _p(pos), _p(mass), self.p, self.eps, self.theta_max,
                            self.softening, nthreads, _p(acc), _p(pot))
         if rc != 0:
             raise ValueError("oracle run failed")
@@ -425,54 +452,54 @@ class OracleFmm:
 
     def moments(self):
         out = np.empty(self.ncells * dim(self.p), dtype=np.complex128)
-        lib().orc_get_moments(self._ctx, out.ctypes.data_as(_pd))
+        self._lib.orc_get_moments(self._ctx, out.ctypes.data_as(_pd))
         return out.reshape(self.ncells, dim(self.p))
 
     def pows(self):
         out = np.empty(self.ncells * (self.p + 1))
-        lib().orc_get_pows(self._ctx, _p(out))
+        self._lib.orc_get_pows(self._ctx, _p(out))
         return out.reshape(self.ncells, self.p + 1)
 
     def f(self):
         out = np.empty(self.n)
-        lib().orc_get_f(self._ctx, _p(out))
+        self._lib.orc_get_f(self._ctx, _p(out))
         return out
 
     def fmin(self):
         out = np.empty(self.ncells)
-        lib().orc_get_fmin(self._ctx, _p(out))
+        self._lib.orc_get_fmin(self._ctx, _p(out))
         return out
 
     def m2l_pairs(self):
-        k = lib().orc_n_m2l(self._ctx)
+        k = self._lib.orc_n_m2l(self._ctx)
         out = np.empty((k, 2), np.uint32)
         if k:
-            lib().orc_get_m2l(self._ctx, _p(out, _pu32))
+            self._lib.orc_get_m2l(self._ctx, _p(out, _pu32))
         return out
 
     def p2p_pairs(self):
-        k = lib().orc_n_p2p(self._ctx)
+        k = self._lib.orc_n_p2p(self._ctx)
         out = np.empty((k, 2), np.uint32)
         if k:
-            lib().orc_get_p2p(self._ctx, _p(out, _pu32))
+            self._lib.orc_get_p2p(self._ctx, _p(out, _pu32))
         return out
 
     def locals_walk(self):
         out = np.empty(self.ncells * dim(self.p), dtype=np.complex128)
-        lib().orc_get_locals_walk(self._ctx, out.ctypes.data_as(_pd))
+        self._lib.orc_get_locals_walk(self._ctx, out.ctypes.data_as(_pd))
         return out.reshape(self.ncells, dim(self.p))
 
     def locals(self):
         out = np.empty(self.ncells * dim(self.p), dtype=np.complex128)
-        lib().orc_get_locals(self._ctx, out.ctypes.data_as(_pd))
+        self._lib.orc_get_locals(self._ctx, out.ctypes.data_as(_pd))
         return out.reshape(self.ncells, dim(self.p))
 
     def near(self):
         out = np.empty((self.n, 4))
-        lib().orc_get_near(self._ctx, _p(out))
+        self._lib.orc_get_near(self._ctx, _p(out))
         return out
 
     def times(self):
         out = np.empty(5)
-        lib().orc_get_times(self._ctx, _p(out))
+        self._lib.orc_get_times(self._ctx, _p(out))
         return dict(zip(["moments", "prepass", "walk", "l2l", "l2p"], out))

@@ -1,0 +1,310 @@
+"""GPU tests at the benchmark configurations, through the C ABI.
+
+  * the mixed FP32-bulk form against the oracle's FP64 FMM and the FP64
+    direct sum (p in {6, 8, 10}, theta 0.5, softening 1e-3, leaf 32 / 128);
+  * the Plummer workload of N = 100,000 (p = 8, theta 0.5, leaf 32,
+    softening 1e-3, FP64): tree, Morton order and interaction lists
+    bit-exact, forces to the oracle bar;
+  * the clustered workload (Hernquist halos in a uniform cube, softening
+    1e-4 with the softening guard) at 262,144 bodies: FP64 lists and forces
+    against the oracle; ten mixed-form KDK steps replayed from a graph equal
+    the eager steps bit for bit and meet the 1e-5 error bar;
+  * a 256-system mixed-form ensemble (three virtual tree levels, systems of
+    2k to 400k bodies, one with near-duplicate bodies whose ensemble keys
+    tie): every system bit-equal to its standalone run, samples against the
+    oracle;
+  * tree and Morton order bit-exact at N = 2,100,000 (sorts and scans over
+    more than 256 tiles);
+  * the host samplers and initial conditions of the library, bit for bit
+    against the oracle, on the GPU host;
+  * the full energy gate of crates/fmm/tests/energy_gate.rs:254-290.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = 16  # oracle threads for the larger cases
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import fmms_amd
+
+    return fmms_amd
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+class Sys:
+    """An oracle initial condition (pos, vel, mass) with the product's view."""
+
+    def __init__(self, t):
+        self.pos, self.vel, self.mass = t
+
+    @property
+    def n(self):
+        return self.mass.shape[0]
+
+
+def oic(fn, *a, **kw):
+    return Sys(fn(*a, **kw))
+
+
+def rel_stats(a, a_ref):
+    rel = np.linalg.norm(a - a_ref, axis=1) / np.linalg.norm(a_ref, axis=1)
+    return float(np.median(rel)), float(np.percentile(rel, 99)), float(rel.max())
+
+
+def same_lists(g, orc):
+    m2l, p2p = g.lists()
+    assert sorted(map(tuple, m2l)) == sorted(map(tuple, orc.m2l_pairs())), "M2L list"
+    assert sorted(map(tuple, p2p)) == sorted(map(tuple, orc.p2p_pairs())), "P2P list"
+
+
+def check_forces(a, pot, a_ref, pot_ref, rel=1e-11):
+    an = np.linalg.norm(a_ref, axis=1)
+    da = np.linalg.norm(a - a_ref, axis=1)
+    assert np.all(da <= rel * an + 1e-13 * an.max()), (da / np.maximum(an, 1e-300)).max()
+    assert np.all(np.abs(pot - pot_ref) <= rel * np.abs(pot_ref) + 1e-13 * np.abs(pot_ref).max())
+
+
+# Mixed form against the FP64 oracle: FP32 pair terms and FP32 M2L on scaled
+# operands keep ~7 significant digits per term; bars on the relative
+# difference to the oracle's FP64 FMM (the same approximation in FP64).
+MIXED_P99, MIXED_MED, MIXED_MAX = 2e-6, 5e-7, 1e-4
+
+
+@pytest.mark.parametrize("p", [6, 8, 10])
+@pytest.mark.parametrize("n_leaf", [32, 128])
+def test_mixed_form_against_oracle(fa, p, n_leaf):
+    s = oic(O.ic_plummer, 20000, rmax=20.0, seed=91)
+    pos, mass = s.pos, s.mass
+    kw = dict(p=p, eps=math.inf, n_leaf=n_leaf, theta_max=0.5, softening=1e-3)
+    say(f"[mixed vs oracle] N=20000 p={p} n_leaf={n_leaf}")
+    orc = O.OracleFmm(pos, mass, **kw)
+    a_ref, pot_ref = orc.run(pos, mass, nthreads=THREADS)
+    g = fa.Fmm(None, None, precision="fp32bulk", **kw)
+    r = g.accel(pos, mass)
+    same_lists(g, orc)
+    med, p99, mx = rel_stats(r.accel, a_ref)
+    dpot = np.abs(r.pot - pot_ref) / np.abs(pot_ref)
+    say(f"  vs oracle FP64: median {med:.2e} p99 {p99:.2e} max {mx:.2e}; pot max {dpot.max():.2e}")
+    assert med <= MIXED_MED and p99 <= MIXED_P99 and mx <= MIXED_MAX
+    assert dpot.max() <= 5e-6
+    # against the FP64 direct sum: no worse than the oracle's own FP64 FMM
+    ids = O.sample_ids(len(mass), 1000, seed=7)
+    d_ref, _, _ = O.direct_targets(pos, mass, ids, softening=1e-3, nthreads=THREADS)
+    e_orc = fa.rel_errors(a_ref[ids], d_ref)
+    e_mix = fa.rel_errors(r.accel[ids], d_ref)
+    say(f"  vs direct: oracle p99 {np.percentile(e_orc, 99):.2e}, mixed p99 "
+        f"{np.percentile(e_mix, 99):.2e}")
+    assert np.percentile(e_mix, 99) <= 1.05 * np.percentile(e_orc, 99) + 5e-7
+
+
+def test_plummer_100k_bit_exact(fa):
+    """The CPU-runnable Plummer workload: N = 100,000 (radii <= 20), p = 8,
+    theta = 0.5, leaf 32, softening 1e-3, FP64."""
+    s = oic(O.ic_plummer, 100000, rmax=20.0, seed=42)
+    pos, mass = s.pos, s.mass
+    ppos, pmass = fa.plummer_truncated(100000, rmax=20.0, seed=42)
+    assert np.array_equal(bits(ppos), bits(pos)) and np.array_equal(bits(pmass), bits(mass))
+    kw = dict(p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+    say("[plummer 100k] tree, order, lists, forces vs oracle")
+    orc = O.OracleFmm(pos, mass, **kw)
+    a_ref, pot_ref = orc.run(pos, mass, nthreads=THREADS)
+    g = fa.Fmm(pos, mass, **kw)
+    r = g.run(pos, mass)
+    tg, to = g.tree(), orc.tree()
+    for k in ("order", "keys", "first", "count", "children", "leaf", "level"):
+        assert np.array_equal(getattr(tg, k), getattr(to, k)), k
+    for k in ("cen", "mass", "rho"):
+        assert np.array_equal(bits(getattr(tg, k)), bits(getattr(to, k))), k
+    same_lists(g, orc)
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
+    ids = fa.sample_ids(len(mass), 2000, 7)
+    d_ref, _, _ = fa.direct_sampled(pos, mass, ids, softening=1e-3)
+    rel = fa.rel_errors(r.accel[ids], d_ref)
+    say(f"  vs direct (2000 targets): median {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    assert np.percentile(rel, 99) <= 1e-5
+
+
+def test_halos_cube_262k(fa):
+    """Clustered workload at reduced N: 64 Hernquist halos (90 % of the mass,
+    masses over two decades, scale radii 0.01-0.1) in a uniform unit cube,
+    softening 1e-4 with the softening guard 1e-4, p = 8, theta = 0.5."""
+    n = 262144
+    s = fa.ic_halos_cube(n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
+                         rmax_over_a=20.0, seed=2024)
+    so = oic(O.ic_halos_cube, n, nhalo=64, halo_frac=0.9, a_min=0.01, a_max=0.1, mass_decades=2.0,
+                         rmax_over_a=20.0, seed=2024)
+    assert np.array_equal(bits(s.pos), bits(so.pos)) and np.array_equal(bits(s.vel), bits(so.vel))
+    kw = dict(p=8, eps=math.inf, n_leaf=64, theta_max=0.5, softening=1e-4, soft_mac=1e-4)
+    say(f"[halos] N={n} FP64 lists and forces vs oracle")
+    orc = O.OracleFmm(s.pos, s.mass, **kw)
+    a_ref, pot_ref = orc.run(s.pos, s.mass, nthreads=THREADS)
+    g = fa.Fmm(s.pos, s.mass, **kw)
+    r = g.run(s.pos, s.mass)
+    same_lists(g, orc)
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
+    # mixed form against the oracle at the same positions
+    rm = fa.Fmm(None, None, precision="fp32bulk", **kw).accel(s.pos, s.mass)
+    med, p99, mx = rel_stats(rm.accel, a_ref)
+    say(f"  mixed vs oracle: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert med <= MIXED_MED and p99 <= MIXED_P99 and mx <= MIXED_MAX
+    # ten KDK steps at dt = 1e-4, rebuilt every 10 steps (drift monitor on):
+    # graph replay equals eager bit for bit
+    say("[halos] 10 mixed KDK steps, graph vs eager")
+    out = []
+    for graph in (False, True):
+        gm = fa.Fmm(None, None, precision="fp32bulk", **kw)
+        it = fa.Integrator(gm, s.pos, s.vel, s.mass)
+        it.step(1e-4, steps=0)
+        it.step(1e-4, steps=10, refresh_every=10, graph=graph)
+        out.append(it.state())
+        st = gm.leapfrog_stats()
+        say(f"  graph={graph}: {st}")
+    for a, b in zip(*out):
+        assert np.array_equal(bits(a), bits(b))
+    x, _, a, _ = out[1]
+    ids = fa.sample_ids(n, 2048, 7)
+    d_ref, _, _ = fa.direct_sampled(x, s.mass, ids, softening=1e-4)
+    rel = fa.rel_errors(a[ids], d_ref)
+    say(f"  after 10 steps vs direct: median {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e}")
+    assert np.percentile(rel, 99) <= 1e-5
+
+
+def _ensemble_256():
+    """254 systems of 2,048-8,192 bodies (half Plummer, half disk) and two of
+    300k-400k, plus near-duplicate bodies in the first large one: 256 systems,
+    so the forest has three virtual levels and ensemble keys keep 18 bits per
+    axis."""
+    rng = np.random.default_rng(256)
+    systems = []
+    for i in range(254):
+        n = int(np.exp(rng.uniform(np.log(2048), np.log(8192))))
+        if i % 2 == 0:
+            systems.append(oic(O.ic_plummer, n, rmax=20.0, seed=1000 + i))
+        else:
+            systems.append(oic(O.ic_disk, n, rd=1.0, zd=0.05, rmax=10.0, m_central=0.1, seed=2000 + i))
+    big = oic(O.ic_plummer, 320000, rmax=20.0, seed=77)
+    # 2,000 bodies moved to within 1e-9 of others: same top 18 key bits per
+    # axis, different standalone keys
+    src = rng.choice(big.n // 2, 2000, replace=False)
+    big.pos[big.n // 2 + np.arange(2000)] = big.pos[src] + 1e-9 * rng.standard_normal((2000, 3))
+    systems.insert(100, big)
+    systems.insert(200, oic(O.ic_disk, 400000, rd=1.0, zd=0.05, rmax=10.0, m_central=0.1, seed=78))
+    return systems
+
+
+def test_ensemble_256_mixed_equals_standalone(fa):
+    systems = _ensemble_256()
+    assert len(systems) == 256
+    pos = np.concatenate([s.pos for s in systems])
+    mass = np.concatenate([s.mass for s in systems])
+    kw = dict(p=8, eps=math.inf, n_leaf=64, theta_max=0.5, softening=1e-3, soft_mac=1e-4,
+              precision="fp32bulk")
+    say(f"[ensemble 256] {len(mass)} bodies, mixed form")
+    g = fa.Fmm(None, None, **kw)
+    off = g.set_systems([s.n for s in systems])
+    r = g.accel(pos, mass)
+    # the near-duplicates tie in the ensemble keys: the two-phase sort keeps
+    # their standalone order
+    one = fa.Fmm(None, None, **kw)
+    rb = one.accel(systems[100].pos, systems[100].mass)
+    keys = one.tree().keys
+    ties = np.count_nonzero((keys[1:] >> 9) == (keys[:-1] >> 9)) - np.count_nonzero(keys[1:] == keys[:-1])
+    assert ties >= 100, ties
+    a, b = int(off[100]), int(off[101])
+    assert np.array_equal(bits(r.accel[a:b]), bits(rb.accel))
+    assert np.array_equal(bits(r.pot[a:b]), bits(rb.pot))
+    for i, s in enumerate(systems):
+        ri = fa.Fmm(None, None, **kw).accel(s.pos, s.mass)
+        a, b = int(off[i]), int(off[i + 1])
+        assert np.array_equal(bits(r.accel[a:b]), bits(ri.accel)), f"system {i}"
+        assert np.array_equal(bits(r.pot[a:b]), bits(ri.pot)), f"system {i}"
+    # samples against the FP64 oracle, including both large systems
+    for i in (0, 1, 100, 200):
+        s = systems[i]
+        orc = O.OracleFmm(s.pos, s.mass, **{k: v for k, v in kw.items() if k != "precision"})
+        a_ref, _ = orc.run(s.pos, s.mass, nthreads=THREADS)
+        a, b = int(off[i]), int(off[i + 1])
+        med, p99, mx = rel_stats(r.accel[a:b], a_ref)
+        say(f"  system {i} (n={s.n}) vs oracle: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+        assert med <= MIXED_MED and p99 <= MIXED_P99 and mx <= MIXED_MAX
+
+
+def test_tree_2m_bit_exact(fa):
+    """N = 2,100,000: the key sort and the scans run over more than 256 tiles
+    (two-level scan, XCD-ordered sort tiles)."""
+    n = 2100000
+    pos, mass = O.plummer(n, seed=5)
+    say(f"[tree] N={n} bit-exact vs oracle")
+    orc = O.OracleFmm(pos, mass, p=4, eps=1e-3, n_leaf=24)
+    g = fa.Fmm(pos, mass, p=4, eps=1e-3, n_leaf=24)
+    tg, to = g.tree(), orc.tree()
+    for k in ("order", "keys", "first", "count", "children", "leaf", "level"):
+        assert np.array_equal(getattr(tg, k), getattr(to, k)), k
+    for k in ("cen", "mass", "rho"):
+        assert np.array_equal(bits(getattr(tg, k)), bits(getattr(to, k))), k
+
+
+def test_host_samplers_match_oracle_on_gpu_host(fa):
+    """The library's host samplers and initial conditions (models.rs:13-82
+    and their extensions) equal the oracle's bit for bit on the GPU host."""
+    say("[samplers] product vs oracle")
+    for fn_p, fn_o in ((fa.plummer, O.plummer), (fa.hernquist, O.hernquist)):
+        a, b = fn_p(5000, center=(0.5, -1.0, 2.0), seed=9), fn_o(5000, center=(0.5, -1.0, 2.0), seed=9)
+        assert np.array_equal(bits(a[0]), bits(b[0])) and np.array_equal(bits(a[1]), bits(b[1]))
+    cases = [(fa.ic_plummer, O.ic_plummer, (5000,), dict(rmax=20.0, seed=3)),
+             (fa.ic_hernquist, O.ic_hernquist, (5000,), dict(a=0.5, rmax=30.0, seed=4)),
+             (fa.ic_disk, O.ic_disk, (5000,), dict(m_central=0.1, sigma=0.02, seed=5)),
+             (fa.ic_halos_cube, O.ic_halos_cube, (20000,), dict(seed=6))]
+    for fp, fo, args, kw in cases:
+        a, b = fp(*args, **kw), Sys(fo(*args, **kw))
+        for k in ("pos", "vel", "mass"):
+            assert np.array_equal(bits(getattr(a, k)), bits(getattr(b, k))), (fp.__name__, k)
+
+
+def test_energy_gate_full(fa):
+    """crates/fmm/tests/energy_gate.rs:254-290: N = 10^6 Plummer (seed 42),
+    p = 8, MAC eps 1e-5, leaf 16, dt = 1e-3, 10^3 steps, tree rebuilt every
+    step, virialised warm start. Gates: |drift| < 1e-3, max deviation < 1e-3,
+    secular slope < 1e-6 per step (least squares of E over the steps, relative
+    to |E0|), relative momentum < 1e-12. The device integrates kick-drift-kick
+    (sim.rs:539-594)."""
+    n, steps, dt = 1_000_000, 1000, 1e-3
+    pos, mass = fa.plummer(n, seed=42)
+    g = fa.Fmm(pos, mass, p=8, eps=1e-5, n_leaf=16)
+    r0 = g.run(pos, mass)
+    vel = O.warm_start(pos, mass, r0.pot)
+    e0 = O.kinetic(mass, vel) + O.potential_energy(mass, r0.pot)
+    it = fa.Integrator(g, pos, vel, mass)
+    it.step(dt, steps=0)
+    es = np.empty(steps)
+    say(f"[energy gate] N={n} steps={steps} dt={dt} E0={e0:.8e}")
+    for k in range(steps):
+        it.step(dt, steps=1, refresh_every=1)
+        v = it.vel.download()
+        pot = it.pot.download()
+        es[k] = O.kinetic(mass, v) + O.potential_energy(mass, pot)
+        if (k + 1) % 100 == 0:
+            say(f"  step {k + 1}: dE/E0 = {(es[k] - e0) / abs(e0):+.3e}")
+    t = np.arange(1, steps + 1, dtype=float)
+    slope = np.polyfit(t, es, 1)[0] / abs(e0)
+    drift = (es[-1] - e0) / abs(e0)
+    max_dev = max(abs(es.min() - e0), abs(es.max() - e0)) / abs(e0)
+    mom = O.momentum_rel(mass, it.vel.download())
+    say(f"  drift {drift:.3e} max_dev {max_dev:.3e} slope {slope:.3e}/step momentum {mom:.3e}")
+    assert abs(drift) < 1e-3 and max_dev < 1e-3 and abs(slope) < 1e-6 and mom < 1e-12

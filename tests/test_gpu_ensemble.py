@@ -48,43 +48,49 @@ def make_systems(fa, k, seed):
     return out
 
 
-PARAMS = [(8, 0.5, 1e-3, 32), (6, 0.6, 0.0, 16), (16, 0.6, 1e-3, 32)]
+PARAMS = [(8, 0.5, 1e-3, 32, "fp64"), (6, 0.6, 0.0, 16, "fp64"), (16, 0.6, 1e-3, 32, "fp64"),
+          (8, 0.5, 1e-3, 32, "fp32bulk"), (10, 0.6, 1e-2, 64, "fp32bulk")]
 
 
-@pytest.mark.parametrize("p,theta,soft,n_leaf", PARAMS)
+@pytest.mark.parametrize("p,theta,soft,n_leaf,precision", PARAMS)
 @pytest.mark.parametrize("k", [1, 5, 13, 70])
-def test_ensemble_equals_standalone(fa, k, p, theta, soft, n_leaf):
+def test_ensemble_equals_standalone(fa, k, p, theta, soft, n_leaf, precision):
     if k == 70 and p == 16:
         pytest.skip("covered at smaller k")
     systems = make_systems(fa, k, seed=k)
-    say(f"[ensemble] k={k} p={p} theta={theta} soft={soft} n={sum(s.n for s in systems)}")
+    say(f"[ensemble] k={k} p={p} theta={theta} soft={soft} {precision} "
+        f"n={sum(s.n for s in systems)}")
     pos = np.concatenate([s.pos for s in systems])
     mass = np.concatenate([s.mass for s in systems])
-    g = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=soft)
+    g = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta, softening=soft,
+               precision=precision)
     off = g.set_systems([s.n for s in systems])
     r = g.accel(pos, mass)
     for i, s in enumerate(systems):
         one = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=theta,
-                     softening=soft)
+                     softening=soft, precision=precision)
         ri = one.accel(s.pos, s.mass)
         a, b = int(off[i]), int(off[i + 1])
         assert np.array_equal(bits(r.accel[a:b]), bits(ri.accel)), f"system {i}"
         assert np.array_equal(bits(r.pot[a:b]), bits(ri.pot)), f"system {i}"
 
 
-def test_ensemble_leapfrog_equals_standalone(fa):
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_ensemble_leapfrog_equals_standalone(fa, precision):
     systems = make_systems(fa, 9, seed=5)
-    say("[ensemble] leapfrog, 3 steps, rebuild every step")
+    say(f"[ensemble] leapfrog, 3 steps, rebuild every step, {precision}")
     pos = np.concatenate([s.pos for s in systems])
     vel = np.concatenate([s.vel for s in systems])
     mass = np.concatenate([s.mass for s in systems])
-    g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+    g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3,
+               precision=precision)
     off = g.set_systems([s.n for s in systems])
     it = fa.Integrator(g, pos, vel, mass)
     it.step(1e-3, steps=3)
     x, v, a, _ = it.state()
     for i, s in enumerate(systems):
-        one = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+        one = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3,
+                     precision=precision)
         it1 = fa.Integrator(one, s.pos, s.vel, s.mass)
         it1.step(1e-3, steps=3)
         x1, v1, a1, _ = it1.state()

@@ -123,13 +123,15 @@ def test_leapfrog_matches_oracle_trajectory(fa):
     assert np.abs(pot - potr).max() <= 1e-10 * np.abs(potr).max()
 
 
-def test_graph_replay_equals_eager_steps(fa):
+@pytest.mark.parametrize("precision", ["fp64", "fp32bulk"])
+def test_graph_replay_equals_eager_steps(fa, precision):
     s = fa.ic_plummer(6000, rmax=20.0, seed=63)
     dt = 1e-3
-    say("[leapfrog] graph replay vs eager, refresh every 3")
+    say(f"[leapfrog] graph replay vs eager, refresh every 3, {precision}")
     runs = []
     for graph in (False, True):
-        g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+        g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3,
+                   precision=precision)
         it = fa.Integrator(g, s.pos, s.vel, s.mass)
         states = []
         for k in range(7):  # one call per step, host copies between replays
@@ -145,7 +147,8 @@ def test_graph_replay_equals_eager_steps(fa):
     # several steps in one call replay the captured graph back to back
     outs = []
     for graph in (False, True):
-        g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3)
+        g = fa.Fmm(None, None, p=8, eps=math.inf, n_leaf=32, theta_max=0.5, softening=1e-3,
+                   precision=precision)
         it = fa.Integrator(g, s.pos, s.vel, s.mass)
         it.step(dt, steps=9, refresh_every=4, graph=graph)
         outs.append(it.state())
