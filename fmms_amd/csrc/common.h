@@ -13,6 +13,11 @@ struct cplx {
 
 // Component-wise complex ops in the order of crates/fmm harmonic.rs:22-74.
 // Files that need bit-exact results are compiled with -ffp-contract=off.
+// FP32 complex of the mixed form (radius-scaled expansions).
+struct cplxf {
+  float re, im;
+};
+
 __device__ __forceinline__ cplx cz() { return cplx{0.0, 0.0}; }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }

@@ -109,6 +109,7 @@ struct fmmh_ctx {
   Buf far, p2p_work, walkrec;
   Buf pl_n, pl_items, pl_reds, pl_partial;  // P2P work plan
   Buf m2l_lam, m2l_m32;                     // mixed form: FP32 M2L scales, scaled moments
+  Buf loc32;                                // mixed form: FP32 locals scaled by lam^n
   uint32_t p_nitems = 0, p_nsplit = 0;
   Buf rt_sw, rt_swt, rt_din, rt_dout, rt_fin, rt_fout, rt_fact;
   Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; L, G, H per cell
@@ -128,7 +129,7 @@ struct fmmh_ctx {
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
-                  &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
+                  &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &loc32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
     bufs.assign(std::begin(all), std::end(all));
@@ -513,6 +514,7 @@ struct fmmh_ctx {
       if (prm.precision == FMMH_PREC_FP32BULK) {
         ensure(m2l_lam, (size_t)nc * sizeof(double));
         ensure(m2l_m32, (size_t)nc * m2l32_moment_bytes(p));
+        ensure(loc32, (size_t)nc * m2l32_moment_bytes(p));
       }
     } else if (p >= 20) {
       rot_tables();
@@ -550,6 +552,9 @@ struct fmmh_ctx {
                   nsplit, (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
   }
 
+  // The mixed form's FP32 M2L, L2L and L2P (templated orders only).
+  bool mixed32() const { return prm.precision == FMMH_PREC_FP32BULK && prm.p <= TEMPLATED_MAX_P; }
+
   // M2L (+ reduce), L2L, L2P and the fused P2P + scatter into acc / pot.
   // Launches only: safe to capture into a graph once the lists exist.
   void far_near(double* acc, double* pot) {
@@ -582,16 +587,22 @@ struct fmmh_ctx {
                        (const double*)ctrace.p, soft2, p, loc, lap, s);
     }
     mark(ST_L2L);
-    for (int L = 1; L <= (int)max_level; L++) {
+    const bool l32 = mixed32();
+    for (int L = l32 ? 0 : 1; L <= (int)max_level; L++) {
       const uint32_t* lc = (uint32_t*)lvl_cells.p + lvl_off[L];
       const uint32_t cnt = lvl_off[L + 1] - lvl_off[L];
-      if (!gen)
+      if (l32)
+        launch_l2l32(p, lc, cnt, c, (const double*)m2l_lam.p, loc, loc32.p, s);
+      else if (!gen)
         launch_l2l(p, lc, cnt, c, loc, s);
       else
         launch_l2l_gen(p, lc, cnt, c, loc, s);
     }
     mark(ST_L2P);
-    if (!gen)
+    if (l32)
+      launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, lap, (double4*)far.p, s,
+                 (const double*)m2l_lam.p, loc32.p);
+    else if (!gen)
       launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, lap, (double4*)far.p, s);
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
@@ -1211,6 +1222,16 @@ int fmmh_get_moments(fmmh_ctx* ctx, double* out) {
 }
 int fmmh_get_locals(fmmh_ctx* ctx, double* out) {
   if (!ctx) return FMMH_E_STATE;
+  if (ctx->evaluated && ctx->mixed32()) {  // FP64 view of the scaled FP32 locals
+    int rc = guarded(ctx, [&] {
+      const size_t bytes = (size_t)ctx->nc * hdim(ctx->prm.p) * 16;
+      ctx->ensure(ctx->direct_out, bytes);
+      launch_loc32_export(ctx->prm.p, ctx->nc, (const double*)ctx->m2l_lam.p, ctx->loc32.p,
+                          (cplx*)ctx->direct_out.p, ctx->s);
+    });
+    if (rc) return rc;
+    return get_dev(ctx, out, ctx->direct_out.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
+  }
   return get_dev(ctx, out, ctx->locals.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
 }
 int fmmh_get_pows(fmmh_ctx* ctx, double* out) {

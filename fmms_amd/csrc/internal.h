@@ -62,6 +62,10 @@ void launch_bbox_sys(const double* pos, const uint64_t* off, uint32_t nsys, BBox
 void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,
                      uint32_t nsys, int vlev, const BBox* bbs, uint64_t* keys, uint32_t* vals,
                      uint32_t* flags, hipStream_t s);
+// Ensemble keys on bodies sorted by standalone key: system prefix over the
+// key's top 63 - 3 vlev bits (then sorted stably on the prefix bits).
+void launch_keys_prefix(uint64_t* keys, const uint32_t* ids, size_t n, const uint64_t* off,
+                        uint32_t nsys, int vlev, hipStream_t s);
 void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb,
                  uint64_t* keys, uint32_t* vals, uint32_t* flags, hipStream_t s);
 void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
@@ -185,8 +189,17 @@ void launch_l2p_gen(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells 
 // ---------------------------------------------------------- downward.hip
 bool launch_l2l(int p, const uint32_t* cells, uint32_t cnt, DevCells c, cplx* locals,
                 hipStream_t s);
+// loc32 non-null: the mixed form's scaled FP32 locals (with their scales lam).
 bool launch_l2p(int p, uint32_t n, const uint32_t* leaf_of_slot, DevCells c, DevBodies b,
-                const cplx* locals, const double* lap, double4* far, hipStream_t s);
+                const cplx* locals, const double* lap, double4* far, hipStream_t s,
+                const double* lam = nullptr, const void* loc32 = nullptr);
+// Mixed form: locals scaled by lam^n in FP32 (m >= 0 half, loc32 [nc] x
+// hnpos(p) complex floats): one level's cells from their own FP64 locals and
+// their parents' scaled locals; and the FP64 view of them.
+bool launch_l2l32(int p, const uint32_t* cells, uint32_t cnt, DevCells c, const double* lam,
+                  const cplx* own, void* loc32, hipStream_t s);
+bool launch_loc32_export(int p, uint32_t nc, const double* lam, const void* loc32, cplx* out,
+                         hipStream_t s);
 
 // --------------------------------------------------------------- soft.hip
 // Trace terms of the softened far field (lap may be null: eps = 0).

@@ -31,11 +31,6 @@ __device__ __forceinline__ void cmac(cplx& f, double ar, double ai, double br, d
   f.im = fma(ai, br, f.im);
 }
 
-// FP32 complex for the mixed form's M2L (radius-scaled operands).
-struct cplxf {
-  float re, im;
-};
-
 __device__ __forceinline__ void cmac(cplxf& f, float ar, float ai, float br, float bi) {
   f.re = fmaf(ar, br, f.re);
   f.re = fmaf(-ai, bi, f.re);

@@ -201,6 +201,15 @@ __device__ __forceinline__ void p2p_tile32(const float4* __restrict__ tile, uint
   }
 }
 
+// Compensated FP64 accumulation (Knuth's TwoSum: s + v = t + e exactly, e
+// kept in c): the mixed form's final per-body sums.
+__device__ __forceinline__ void acc2(double& s, double& c, double v) {
+  const double t = s + v;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (v - bp);
+  s = t;
+}
+
 __device__ __forceinline__ float4 rel32(double4 v, double cx, double cy, double cz) {
   return make_float4((float)(v.x - cx), (float)(v.y - cy), (float)(v.z - cz), (float)v.w);
 }
@@ -238,7 +247,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(const uint4* __restrict__ items,
     const bool tvalid = ti < nt;
     const uint32_t si = ft + t0 + (tvalid ? ti : 0);
     const float4 me = rel32(body[si], cx, cy, cz);
-    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;  // FP64 sums of the FP32 tile sums
     for (uint32_t s0 = 0; own && s0 < nt_all; s0 += 64) {
       __syncthreads();
       const bool in = s0 + lane < nt_all;
@@ -278,22 +287,33 @@ __global__ void __launch_bounds__(64) k_p2p_f32(const uint4* __restrict__ items,
         __syncthreads();
       }
     }
+    // the final per-body sum (sub-lane partials, then the far field) is
+    // compensated
+    double kx = 0.0, ky = 0.0, kz = 0.0, kp = 0.0;
     for (uint32_t o = ntp; o < 64; o <<= 1) {
-      ax += __shfl_xor(ax, (int)o, 64);
-      ay += __shfl_xor(ay, (int)o, 64);
-      az += __shfl_xor(az, (int)o, 64);
-      ph += __shfl_xor(ph, (int)o, 64);
+      acc2(ax, kx, __shfl_xor(ax, (int)o, 64));
+      acc2(ay, ky, __shfl_xor(ay, (int)o, 64));
+      acc2(az, kz, __shfl_xor(az, (int)o, 64));
+      acc2(ph, kp, __shfl_xor(ph, (int)o, 64));
+      kx += __shfl_xor(kx, (int)o, 64);
+      ky += __shfl_xor(ky, (int)o, 64);
+      kz += __shfl_xor(kz, (int)o, 64);
+      kp += __shfl_xor(kp, (int)o, 64);
     }
     if (sub == 0 && tvalid) {
-      if (it.w == P2P_DIRECT) {
+      if (it.w == P2P_DIRECT) {  // the final near + far sum, compensated
         const double4 fv = far[si];
         const uint32_t id = order[si];
-        accel[3 * (size_t)id] = ax + fv.x;
-        accel[3 * (size_t)id + 1] = ay + fv.y;
-        accel[3 * (size_t)id + 2] = az + fv.z;
-        if (pot) pot[id] = ph + fv.w;
+        acc2(ax, kx, fv.x);
+        acc2(ay, ky, fv.y);
+        acc2(az, kz, fv.z);
+        acc2(ph, kp, fv.w);
+        accel[3 * (size_t)id] = ax + kx;
+        accel[3 * (size_t)id + 1] = ay + ky;
+        accel[3 * (size_t)id + 2] = az + kz;
+        if (pot) pot[id] = ph + kp;
       } else {
-        partial[(size_t)it.w + t0 + ti] = make_double4(ax, ay, az, ph);
+        partial[(size_t)it.w + t0 + ti] = make_double4(ax + kx, ay + ky, az + kz, ph + kp);
       }
     }
   }
@@ -455,6 +475,9 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_emit(const uint32_t* __rest
 
 // Split leaves: far field + the items' partial sums in item order, scattered
 // to particle-id order.
+// COMP (the mixed form): the item sums and the far field are added with
+// compensated (TwoSum) accumulation.
+template <bool COMP>
 __global__ void __launch_bounds__(64) k_p2p_reduce(const uint4* __restrict__ reds,
                                                   const uint32_t* __restrict__ cfirst,
                                                   const uint32_t* __restrict__ ccount,
@@ -467,19 +490,26 @@ __global__ void __launch_bounds__(64) k_p2p_reduce(const uint4* __restrict__ red
   const uint32_t ft = cfirst[r.x], nt = ccount[r.x];
   for (uint32_t t = threadIdx.x; t < nt; t += 64) {
     double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
-    for (uint32_t k = 0; k < r.z; k++) {
-      const double4 v = partial[(size_t)r.y + (size_t)k * nt + t];
-      ax += v.x;
-      ay += v.y;
-      az += v.z;
-      ph += v.w;
+    double kx = 0.0, ky = 0.0, kz = 0.0, kp = 0.0;
+    for (uint32_t k = 0; k <= r.z; k++) {  // the items in order, then the far field
+      const double4 v = k < r.z ? partial[(size_t)r.y + (size_t)k * nt + t] : far[ft + t];
+      if (COMP) {
+        acc2(ax, kx, v.x);
+        acc2(ay, ky, v.y);
+        acc2(az, kz, v.z);
+        acc2(ph, kp, v.w);
+      } else {
+        ax += v.x;
+        ay += v.y;
+        az += v.z;
+        ph += v.w;
+      }
     }
-    const double4 fv = far[ft + t];
     const uint32_t id = order[ft + t];
-    accel[3 * (size_t)id] = ax + fv.x;
-    accel[3 * (size_t)id + 1] = ay + fv.y;
-    accel[3 * (size_t)id + 2] = az + fv.z;
-    if (pot) pot[id] = ph + fv.w;
+    accel[3 * (size_t)id] = ax + kx;
+    accel[3 * (size_t)id + 1] = ay + ky;
+    accel[3 * (size_t)id + 2] = az + kz;
+    if (pot) pot[id] = ph + kp;
   }
 }
 
@@ -521,7 +551,7 @@ void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const ui
   hipLaunchKernelGGL(k_p2p_f32, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
   if (pl.nsplit)
-    hipLaunchKernelGGL(k_p2p_reduce, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
+    hipLaunchKernelGGL(k_p2p_reduce<true>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
                        pl.partial, far, order, accel, pot);
 }
 
@@ -532,7 +562,7 @@ void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2*
   hipLaunchKernelGGL(k_p2p, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first, c.count,
                      body, soft2, far, order, accel, pot, pl.partial);
   if (pl.nsplit)
-    hipLaunchKernelGGL(k_p2p_reduce, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
+    hipLaunchKernelGGL(k_p2p_reduce<false>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
                        pl.partial, far, order, accel, pot);
 }
 

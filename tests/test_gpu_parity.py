@@ -410,9 +410,9 @@ def test_large_theta_mac_properties(fa):
 
 @pytest.mark.parametrize("soft,n_leaf", [(1e-3, 32), (0.0, 96), (1e-2, 64)])
 def test_fp32bulk_form_accuracy(fa, soft, n_leaf):
-    """FP32-bulk form (P2P pair terms in FP32 on leaf-centred positions, M2L
-    in FP32 on radius-scaled operands, FP64 tile, chunk and final sums): 99 %
-    of bodies within 1e-6 of the FP64 path, every
+    """FP32-bulk form (P2P pair terms in FP32 on leaf-centred positions; M2L,
+    L2L and L2P in FP32 on radius-scaled operands; FP64 tile and chunk sums,
+    compensated final sums): 99 % of bodies within 2e-6 of the FP64 path, every
     body within 5e-5 (an unsoftened close pair in a large leaf keeps only the
     FP32 resolution of leaf-centred coordinates), and no worse than the FP64
     path against the FP64 direct sum (the metric's bar)."""
@@ -423,7 +423,7 @@ def test_fp32bulk_form_accuracy(fa, soft, n_leaf):
     r32 = fa.Fmm(None, None, precision="fp32bulk", **kw).accel(pos, mass)
     an = np.linalg.norm(r64.accel, axis=1)
     d = np.linalg.norm(r32.accel - r64.accel, axis=1) / an
-    assert np.percentile(d, 99) < 1e-6 and np.median(d) < 5e-7, (np.percentile(d, 99), np.median(d))
+    assert np.percentile(d, 99) < 2e-6 and np.median(d) < 5e-7, (np.percentile(d, 99), np.median(d))
     assert d.max() < 5e-5, d.max()
     assert np.all(np.abs(r32.pot - r64.pot) <= 5e-6 * np.abs(r64.pot))
     ids = fa.sample_ids(len(mass), 2000, 7)
