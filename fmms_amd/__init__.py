@@ -105,6 +105,8 @@ SIGNATURES = {
     "fmmh_work_counts": (ct.c_int, [_vp, _pu64]),
     "fmmh_p2p_plan_counts": (ct.c_int, [_vp, _pu64]),
     "fmmh_direct_sampled": (ct.c_int, [_sz, _pd, _pd, ct.c_double, _sz, _pu32, _pd, _pd, _pd]),
+    "fmmh_direct_sampled_noalloc": (ct.c_int, [_sz, _vp, _vp, ct.c_double, _sz, _vp, _vp, _vp,
+                                               _vp, _vp]),
     "fmmh_plummer": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_hernquist": (None, [_sz, _pd, ct.c_uint64, _pd, _pd]),
     "fmmh_plummer_trunc": (None, [_sz, _pd, ct.c_double, ct.c_uint64, _pd, _pd]),

@@ -45,6 +45,7 @@ SOURCES = {
     "m2l_p10.hip": "fast",
     "downward.hip": "fast",
     "near.hip": "fast",
+    "direct.hip": "fast",
     "generic_up.hip": "off",
     "generic_far.hip": "fast",
     "fmmh.hip": "off",

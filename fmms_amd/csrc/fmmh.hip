@@ -115,7 +115,7 @@ struct fmmh_ctx {
   Buf ctrace, clap;  // softened far field: q = sum m |x - z|^2; L, G, H per cell
   int rot_p = -1;
   std::vector<char> rot_struct;
-  Buf direct_ids, direct_out;
+  Buf direct_ids, direct_out, direct_part;
   Buf pinned;
 
   fmmh_ctx() {
@@ -130,7 +130,7 @@ struct fmmh_ctx {
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
                   &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
                   &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &loc32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
-                  &rt_fact, &direct_ids, &direct_out, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
+                  &rt_fact, &direct_ids, &direct_out, &direct_part, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
     bufs.assign(std::begin(all), std::end(all));
   }
@@ -1290,24 +1290,23 @@ int fmmh_direct_sampled(size_t n, const double* pos, const double* mass, double 
   return guarded(&ctx, [&] {
     ctx.ensure(ctx.in_pos, n * 24);
     ctx.ensure(ctx.in_mass, n * 8);
+    const uint32_t nsplit = direct_split(n, nt);
     ctx.ensure(ctx.direct_ids, nt * 4);
     ctx.ensure(ctx.direct_out, nt * 40);
+    ctx.ensure(ctx.direct_part, nt * (size_t)nsplit * 40);
     HCHECK(hipMemcpyAsync(ctx.in_pos.p, pos, n * 24, hipMemcpyHostToDevice, ctx.s));
     HCHECK(hipMemcpyAsync(ctx.in_mass.p, mass, n * 8, hipMemcpyHostToDevice, ctx.s));
     HCHECK(hipMemcpyAsync(ctx.direct_ids.p, idx, nt * 4, hipMemcpyHostToDevice, ctx.s));
-    launch_direct_targets((const double*)ctx.in_pos.p, (const double*)ctx.in_mass.p, n,
-                          softening * softening, (const uint32_t*)ctx.direct_ids.p, (uint32_t)nt,
-                          (double*)ctx.direct_out.p, ctx.s);
+    double* out = (double*)ctx.direct_out.p;  // acc [nt][3], pot [nt], f [nt]
+    launch_direct((const double*)ctx.in_pos.p, (const double*)ctx.in_mass.p, n,
+                  softening * softening, (const uint32_t*)ctx.direct_ids.p, (uint32_t)nt, nsplit,
+                  (double*)ctx.direct_part.p, out, out + 3 * nt, out + 4 * nt, ctx.s);
     std::vector<double> h(nt * 5);
-    HCHECK(hipMemcpyAsync(h.data(), ctx.direct_out.p, nt * 40, hipMemcpyDeviceToHost, ctx.s));
+    HCHECK(hipMemcpyAsync(h.data(), out, nt * 40, hipMemcpyDeviceToHost, ctx.s));
     HCHECK(hipStreamSynchronize(ctx.s));
-    for (size_t t = 0; t < nt; t++) {
-      acc[3 * t] = h[5 * t];
-      acc[3 * t + 1] = h[5 * t + 1];
-      acc[3 * t + 2] = h[5 * t + 2];
-      if (pot) pot[t] = h[5 * t + 3];
-      if (f) f[t] = h[5 * t + 4];
-    }
+    std::memcpy(acc, h.data(), nt * 24);
+    if (pot) std::memcpy(pot, h.data() + 3 * nt, nt * 8);
+    if (f) std::memcpy(f, h.data() + 4 * nt, nt * 8);
   });
 }
 

@@ -289,8 +289,14 @@ void launch_drift_check(uint32_t n, const uint32_t* leaf_of_slot, DevCells c, De
                         uint32_t* flag, hipStream_t s);
 
 // ------------------------------------------------------------ direct.hip
-void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,
-                           const uint32_t* ids, uint32_t nt, double* out /*[nt][5]*/,
-                           hipStream_t s);
+// FP64 direct sum at nt sampled targets: each target's sources are cut into
+// nsplit contiguous ranges (one block each, partial sums in part[nt][nsplit][5]),
+// then added in range order into acc [nt][3], pot [nt], f [nt] (pot, f may be
+// null). direct_split picks nsplit for (n, nt) within the allocation-free
+// entry point's scratch, so both entry points sum in the same order.
+uint32_t direct_split(size_t n, size_t nt);
+void launch_direct(const double* pos, const double* mass, size_t n, double soft2,
+                   const uint32_t* ids, uint32_t nt, uint32_t nsplit, double* part, double* acc,
+                   double* pot, double* f, hipStream_t s);
 
 }  // namespace fmmh

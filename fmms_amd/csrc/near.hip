@@ -605,50 +605,4 @@ void launch_p2p_count(const uint2* pairs, uint32_t np, const uint32_t* count,
   hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, out + 1, nb, out);
 }
 
-// ------------------------------------------------------------ direct sum
-// FP64 direct sum at sampled targets (direct.rs:6-31 semantics, optional
-// softening): one 256-thread block per target, fixed-order tree reduction.
-__global__ void __launch_bounds__(256) k_direct_targets(const double* __restrict__ pos,
-                                                       const double* __restrict__ mass, size_t n,
-                                                       double soft2,
-                                                       const uint32_t* __restrict__ ids,
-                                                       double* __restrict__ out) {
-  __shared__ double sh[5][256];
-  const uint32_t id = ids[blockIdx.x];
-  const double xi = pos[3 * (size_t)id], yi = pos[3 * (size_t)id + 1], zi = pos[3 * (size_t)id + 2];
-  double ax = 0, ay = 0, az = 0, ph = 0, f = 0;
-  for (size_t j = threadIdx.x; j < n; j += 256) {
-    if (j == id) continue;
-    double dx = pos[3 * j] - xi, dy = pos[3 * j + 1] - yi, dz = pos[3 * j + 2] - zi;
-    double r2 = dx * dx + dy * dy + dz * dz + soft2;
-    double inv = 1.0 / sqrt(r2);
-    double inv2 = inv * inv;
-    double mj = mass[j];
-    double s3 = mj * inv2 * inv;
-    ax += s3 * dx;
-    ay += s3 * dy;
-    az += s3 * dz;
-    ph += mj * inv;
-    f += mj * inv2;
-  }
-  sh[0][threadIdx.x] = ax;
-  sh[1][threadIdx.x] = ay;
-  sh[2][threadIdx.x] = az;
-  sh[3][threadIdx.x] = ph;
-  sh[4][threadIdx.x] = f;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
-      for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x < 5) out[(size_t)blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-void launch_direct_targets(const double* pos, const double* mass, size_t n, double soft2,
-                           const uint32_t* ids, uint32_t nt, double* out, hipStream_t s) {
-  if (nt == 0) return;
-  hipLaunchKernelGGL(k_direct_targets, dim3(nt), dim3(256), 0, s, pos, mass, n, soft2, ids, out);
-}
-
 }  // namespace fmmh

@@ -212,6 +212,23 @@ int fmmh_p2p_plan_counts(const fmmh_ctx* ctx, uint64_t* out2);
 int fmmh_direct_sampled(size_t n, const double* pos, const double* mass, double softening,
                         size_t nt, const uint32_t* idx, double* acc, double* pot, double* f);
 
+/* The same FP64 direct sum, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Device pointers: pos [n][3], mass [n], idx [nt] (ids out of range give NaN
+ * sums), acc [nt][3]; pot [nt] and f [nt] may be NULL. The kernels are
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting: the results are in acc/pot/f once that stream's
+ * work is done. Partial sums live in a static 8 MiB device array of the
+ * library (one per GPU, held until the process ends), so
+ * nt <= FMMH_DIRECT_NOALLOC_MAX_TARGETS (FMMH_E_PARAM beyond) and calls that
+ * overlap on one GPU race. Each call overwrites every partial it reads.
+ * Results equal fmmh_direct_sampled's bit for bit. */
+#define FMMH_DIRECT_NOALLOC_MAX_TARGETS 209715
+int fmmh_direct_sampled_noalloc(size_t n, const double* d_pos, const double* d_mass,
+                                double softening, size_t nt, const uint32_t* d_idx, double* d_acc,
+                                double* d_pot, double* d_f, void* stream);
+
 /* Initial conditions (models.rs:13-82, seeded SplitMix64), host arrays. */
 void fmmh_plummer(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
 void fmmh_hernquist(size_t n, const double center[3], uint64_t seed, double* pos, double* mass);
