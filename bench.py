@@ -86,37 +86,6 @@ def run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, k):
     return time.perf_counter() - t0, lat, stages
 
 
-def autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot, precision="fp64"):
-    """Fastest (p, n_leaf) over p in --p-grid and n_leaf in --leaf-grid whose
-    relative force error meets the metric's bar (99th percentile <= --max-rel-err
-    against the FP64 direct sum on the sampled targets)."""
-    ids = fa.sample_ids(n, args.samples, 7)
-    a_ref, _, _ = fa.direct_sampled(pos, mass, ids, softening=args.softening)
-    grid_p = [args.p] if args.p_grid is None else [int(x) for x in args.p_grid.split(",")]
-    leaves = [int(x) for x in args.leaf_grid.split(",")]
-    best, tried = None, []
-    for p in grid_p:
-        for nl in leaves:
-            ctx = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=nl, theta_max=args.theta,
-                         softening=args.softening, precision=precision)
-            ctx.accel_device(n, d_pos.ptr, d_mass.ptr, d_acc.ptr, d_pot.ptr)
-            el, _, _ = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, 2)
-            ctx.close()
-            acc = d_acc.download()
-            p99 = float(np.percentile(fa.rel_errors(acc[ids], a_ref), 99))
-            ok = p99 <= args.max_rel_err
-            tried.append({"p": p, "n_leaf": nl, "ms": round(1e3 * el / 2, 3), "rel_p99": p99,
-                          "valid": ok})
-            log(f"[autotune] {precision} p={p} n_leaf={nl}: {1e3 * el / 2:.2f} ms/step, rel p99 {p99:.2e}"
-                f"{'' if ok else ' (above the bar)'}")
-            if ok and (best is None or el < best[2]):
-                best = (p, nl, el)
-    if best is None:  # nothing meets the bar: the most accurate candidate
-        t = min(tried, key=lambda r: r["rel_p99"])
-        best = (t["p"], t["n_leaf"], None)
-    return best[0], best[1], tried
-
-
 def host_cpu():
     """nproc, the CPUs this process may use, and the CPU model."""
     model = None
@@ -169,33 +138,74 @@ def cpu_baseline(pos, mass, args, p, n_leaf, repeats=2):
     }
 
 
+def stage_work(wc, p, precision):
+    """Algorithmic work of one evaluation per stage: (kind, amount, dtype).
+    VALU stages count flops: P2P 22 per directed body-body interaction; M2L,
+    M2M and L2L 8 T(p) per directed pair or edge (T(p) = sum_n (2n+1)(p-n+1)^2
+    complex multiply-adds); P2M 14 dim(p) per body (Upsilon table and the
+    mass-weighted sum); L2P 10 dim(p) + 8 (dim(p) + 2 p^2) per body. Byte
+    stages count the HBM bytes their algorithm must move: tree 336 per body
+    (Morton key in 44, eight sort passes of 12-byte pairs read and written
+    192, gather to slot order 100), lists 16 bytes per directed key per radix
+    pass plus 8 to form it and 8 per pair read, the walk one 64-byte record
+    pair per decided pair, the M2L reduce its chunk partials in and locals
+    out."""
+    n, nc = wc["n"], wc["ncells"]
+    dim = (p + 1) ** 2
+    nf = (p + 1) * (p + 2) // 2
+    m2l, p2p = wc["m2l_pairs"], wc["p2p_pairs"]
+    mixed = precision == "fp32bulk"
+    f32 = "f32" if mixed else "f64"
+    sbits = max(1, int(nc).bit_length())
+    passes = (2 * sbits + 7) // 8
+    keys = 2 * (m2l + p2p)
+    return {
+        "tree": ("hbm", 336.0 * n, None),
+        "upward": ("valu", 14.0 * dim * n + 8.0 * t_full(p) * max(nc - 1, 0), "f64"),
+        "walk": ("hbm", 128.0 * (m2l + p2p), None),
+        "lists": ("hbm", keys * (16.0 * passes + 8.0) + 8.0 * (m2l + p2p), None),
+        "m2l": ("valu", 8.0 * t_full(p) * 2 * m2l, f32),
+        "m2l_reduce": ("hbm", (2.0 * m2l / 32 + nc) * nf * (8 if mixed else 16) + 16.0 * dim * nc,
+                       None),
+        "l2l": ("valu", 8.0 * t_full(p) * max(nc - 1, 0), f32),
+        "l2p": ("valu", (10.0 * dim + 8.0 * (dim + 2 * p * p)) * n, f32),
+        "p2p": ("valu", 22.0 * wc["p2p_interactions"], f32),
+    }
+
+
 def stage_rooflines(stages_ms, wc, p, precision="fp64"):
-    """Achieved rates of the VALU stages (algorithmic flops / stage time)
-    against the peak of the arithmetic they run in: FP64 for the plain form;
-    FP32 for the mixed form's M2L and P2P (beside the published packed peak,
-    the unpacked FP32 ceiling this build can reach)."""
-    work = {"m2l": 8.0 * t_full(p) * 2 * wc["m2l_pairs"], "p2p": 22.0 * wc["p2p_interactions"]}
+    """Achieved rate of every stage (its algorithmic work, stage_work, over
+    its time) against the peak that bounds it: HBM bandwidth for the byte
+    stages; the vector peak of the arithmetic a VALU stage runs in (FP64; for
+    the mixed form's FP32 stages the published packed FP32 peak, with the
+    unpacked FP32 ceiling this build can reach beside it)."""
     out = {}
-    for k, w in work.items():
-        if stages_ms.get(k, 0) > 0:
-            ach = w / (stages_ms[k] * 1e-3) / 1e12
-            if precision == "fp32bulk":
-                out[k] = {"ms": stages_ms[k], "achieved": ach, "dtype": "f32",
-                          "peak": FP32_PEAK_TFLOPS, "frac": ach / FP32_PEAK_TFLOPS,
-                          "peak_unpacked": FP64_PEAK_TFLOPS,
-                          "frac_unpacked": ach / FP64_PEAK_TFLOPS}
-            else:
-                out[k] = {"ms": stages_ms[k], "achieved": ach, "dtype": "f64",
-                          "peak": FP64_PEAK_TFLOPS, "frac": ach / FP64_PEAK_TFLOPS}
+    for k, (kind, w, dt) in stage_work(wc, p, precision).items():
+        ms = stages_ms.get(k, 0)
+        if not ms or ms < 0.02:  # the stage did not run (frozen evaluations skip tree, walk, lists)
+            continue
+        if kind == "hbm":
+            ach = w / (ms * 1e-3) / 1e9
+            out[k] = {"ms": ms, "bound": "hbm", "achieved": ach, "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                      "frac": ach / HBM_PEAK_GBS}
+            continue
+        ach = w / (ms * 1e-3) / 1e12
+        if dt == "f32":
+            out[k] = {"ms": ms, "bound": "valu", "achieved": ach, "unit": "TFLOP/s", "dtype": "f32",
+                      "peak": FP32_PEAK_TFLOPS, "frac": ach / FP32_PEAK_TFLOPS,
+                      "peak_unpacked": FP64_PEAK_TFLOPS, "frac_unpacked": ach / FP64_PEAK_TFLOPS}
+        else:
+            out[k] = {"ms": ms, "bound": "valu", "achieved": ach, "unit": "TFLOP/s", "dtype": "f64",
+                      "peak": FP64_PEAK_TFLOPS, "frac": ach / FP64_PEAK_TFLOPS}
     return out
 
 
 def roofline_line(stage_roof, dom, extra=None):
     """The roofline object of the dominant stage."""
     r = stage_roof[dom]
-    out = {"kernel": dom, "bound": "valu", "dtype": r["dtype"], "achieved": round(r["achieved"], 3),
-           "peak": r["peak"], "unit": "TFLOP/s", "frac": round(r["frac"], 4), "traffic": None,
-           "avg_launch_ms": round(r["ms"], 4)}
+    out = {"kernel": dom, "bound": r["bound"], "dtype": r.get("dtype"),
+           "achieved": round(r["achieved"], 3), "peak": r["peak"], "unit": r["unit"],
+           "frac": round(r["frac"], 4), "traffic": None, "avg_launch_ms": round(r["ms"], 4)}
     if "peak_unpacked" in r:
         out["peak_unpacked"] = r["peak_unpacked"]
         out["frac_unpacked"] = round(r["frac_unpacked"], 4)
@@ -327,6 +337,7 @@ def bench_halos(fa, hip, args):
     fp64 = next((f for f in forms if f["precision"] == "fp64"), None)
     roof_stages = stage_rooflines(best["frozen_ms"], best["wc"], args.p, best["precision"])
     dom = max(roof_stages, key=lambda k: roof_stages[k]["ms"])
+    full_roof = stage_rooflines(best["full_ms"], best["wc"], args.p, best["precision"])
     roof = roofline_line(roof_stages, dom)
     out = {
         "metric": METRIC,
@@ -362,6 +373,7 @@ def bench_halos(fa, hip, args):
         "stages_ms_full": {k: round(v, 4) for k, v in best["full_ms"].items()},
         "stages_ms_frozen": {k: round(v, 4) for k, v in best["frozen_ms"].items()},
         "stage_roofline": roof_stages,
+        "stage_roofline_full": full_roof,
         "work": best["wc"],
         "cpu_baseline": None,
     }
@@ -602,6 +614,13 @@ def main():
     ap.add_argument("--p-grid", default=None,
                     help="expansion orders the autotuner tries (with --n-leaf 0); default 6,8 at "
                          "16M bodies, 6,8,10 below")
+    ap.add_argument("--chunk-grid", default="16,32,64",
+                    help="M2L chunk lengths the autotuner tries (with --n-leaf 0)")
+    ap.add_argument("--item-grid", default="65536,262144,1048576",
+                    help="P2P work-item bounds the autotuner tries (with --n-leaf 0)")
+    ap.add_argument("--m2l-chunk", type=int, default=None, help="fixed M2L chunk (with --n-leaf)")
+    ap.add_argument("--p2p-item-work", type=int, default=None,
+                    help="fixed P2P work-item bound (with --n-leaf)")
     ap.add_argument("--cpu-n", type=int, default=1 << 20,
                     help="bodies of the CPU baseline's sample of the workload (0 = the full N)")
     ap.add_argument("--max-rel-err", type=float, default=1e-5,
@@ -673,15 +692,28 @@ def main():
 
     def measure(precision):
         """Autotune (unless --n-leaf), then time --steps evaluations of one form."""
+        from fmms_amd import autotune
+
         tuned, p = None, args.p
+        tiles = {"m2l_chunk": args.m2l_chunk, "p2p_item_work": args.p2p_item_work}
         if args.n_leaf:
             n_leaf = args.n_leaf
         else:
-            p, n_leaf, tuned = autotune(fa, hip, args, n, pos, mass, d_pos, d_mass, d_acc, d_pot,
-                                        precision)
-        log(f"[bench] {precision}: p={p} n_leaf={n_leaf}")
+            ref = (ids, a_ref) if a_ref is not None else (ids, fa.direct_sampled(
+                pos, mass, ids, softening=args.softening)[0])
+            tuned = autotune.tune(
+                pos, mass, p_grid=[int(x) for x in args.p_grid.split(",")],
+                leaf_grid=[int(x) for x in args.leaf_grid.split(",")],
+                chunk_grid=[int(x) for x in args.chunk_grid.split(",")],
+                item_grid=[int(x) for x in args.item_grid.split(",")], precision=precision,
+                theta=args.theta, softening=args.softening, max_rel_err=args.max_rel_err,
+                device=(d_pos, d_mass, d_acc, d_pot), reference=ref, log=log)
+            p, n_leaf = tuned["p"], tuned["n_leaf"]
+            tiles = {"m2l_chunk": tuned["m2l_chunk"], "p2p_item_work": tuned["p2p_item_work"]}
+        log(f"[bench] {precision}: p={p} n_leaf={n_leaf} tiles={tiles}")
         ctx = fa.Fmm(None, None, p=p, eps=math.inf, n_leaf=n_leaf, theta_max=args.theta,
-                     softening=args.softening, timing=True, precision=precision)
+                     softening=args.softening, timing=True, precision=precision, **tiles)
+        tiles = ctx.tuning()
         run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, max(args.warmup, 1))
         log(f"[bench] {precision}: timing {args.steps} steps")
         elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
@@ -697,7 +729,8 @@ def main():
                    "scaled_rms": float(np.sqrt(np.mean(scaled ** 2))),
                    "scaled_max": float(scaled.max())}
             log(f"[bench] {precision}: rel err median {err['rel_median']:.3e} p99 {err['rel_p99']:.3e}")
-        return {"precision": precision, "p": p, "n_leaf": n_leaf, "elapsed": elapsed, "lat": lat,
+        return {"precision": precision, "p": p, "n_leaf": n_leaf, "tiles": tiles,
+                "elapsed": elapsed, "lat": lat,
                 "stages": {k: float(np.mean(v)) for k, v in stages.items()}, "wc": wc,
                 "acc": acc, "tuned": tuned, "err": err,
                 "value": n * args.steps / elapsed}
@@ -711,8 +744,8 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     stage_roof = stage_rooflines(avg, wc, args.p, best["precision"])
     dom = max(stage_roof, key=lambda k: stage_roof[k]["ms"])
-    work = {"m2l": 8.0 * t_full(args.p) * 2 * wc["m2l_pairs"], "p2p": 22.0 * wc["p2p_interactions"]}
-    roof = roofline_line(stage_roof, dom, {"work_per_launch": work[dom]})
+    roof = roofline_line(stage_roof, dom,
+                         {"work_per_launch": stage_work(wc, args.p, best["precision"])[dom][1]})
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3
     # FETCH_SIZE/WRITE_SIZE runs of this workload (profiles/), when present.
     tag = f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_{best['precision']}"
@@ -769,14 +802,16 @@ def main():
         "data": "synthetic (seeded Plummer sampler, radii truncated at 20)",
         "config": {"workload": tag, "n": n, "p": args.p, "theta": args.theta, "eps_mac": "inf",
                    "softening": args.softening, "n_leaf": n_leaf, "precision": best["precision"],
-                   "form": form_names[best["precision"]],
-                   "selection": (f"fastest form and (p, n_leaf) over p in {{{args.p_grid}}} x "
-                                 f"n_leaf in {{{args.leaf_grid}}} with rel err p99 <= "
-                                 f"{args.max_rel_err:g}")
+                   "form": form_names[best["precision"]], **best["tiles"],
+                   "selection": (f"fmms_amd.autotune: fastest form and (p, n_leaf) over p in "
+                                 f"{{{args.p_grid}}} x n_leaf in {{{args.leaf_grid}}} with rel err "
+                                 f"p99 <= {args.max_rel_err:g}, then (m2l_chunk, p2p_item_work) "
+                                 f"over {{{args.chunk_grid}}} x {{{args.item_grid}}}")
                    if best["tuned"] else "fixed"},
         "error": best["err"],
         "forms": {f["precision"]: {"value": f["value"], "ms_per_step": 1e3 * f["elapsed"] / args.steps,
-                                   "p": f["p"], "n_leaf": f["n_leaf"], "error": f["err"],
+                                   "p": f["p"], "n_leaf": f["n_leaf"], **f["tiles"],
+                                   "error": f["err"],
                                    "stages_ms": {k: round(v, 4) for k, v in f["stages"].items()}}
                   for f in forms},
         "mixed_over_fp64": (best["value"] / fp64["value"]) if fp64 else None,

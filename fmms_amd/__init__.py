@@ -68,6 +68,10 @@ class _LeapOpts(ct.Structure):
                 ("drift_tol", ct.c_double), ("flags", ct.c_uint)]
 
 
+class _Tuning(ct.Structure):
+    _fields_ = [("m2l_chunk", ct.c_int), ("p2p_item_work", ct.c_uint64)]
+
+
 FMMH_LF_ACC_VALID = 1
 FMMH_LF_GRAPH = 2
 
@@ -93,6 +97,9 @@ SIGNATURES = {
     "fmmh_accel_device": (ct.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
     "fmmh_num_cells": (_sz, [_vp]),
     "fmmh_device_bytes": (_sz, [_vp]),
+    "fmmh_default_tuning": (None, [ct.POINTER(_Tuning)]),
+    "fmmh_set_tuning": (ct.c_int, [_vp, ct.POINTER(_Tuning)]),
+    "fmmh_get_tuning": (ct.c_int, [_vp, ct.POINTER(_Tuning)]),
     "fmmh_get_tree": (ct.c_int, [_vp, _pu32, _pu64, _pu32, _pu32, _pi32, _pu8, _pu32, _pd, _pd, _pd]),
     "fmmh_get_list_sizes": (ct.c_int, [_vp, ct.POINTER(_sz), ct.POINTER(_sz)]),
     "fmmh_get_lists": (ct.c_int, [_vp, _pu32, _pu32]),
@@ -180,7 +187,7 @@ class Fmm:
 
     def __init__(self, pos, mass, p, eps, n_leaf, center_mode=CenterMode.Com, theta_max=1.0,
                  softening=0.0, timing=False, hbm_budget_bytes=0.0, precision="fp64",
-                 soft_mac=0.0):
+                 soft_mac=0.0, m2l_chunk=None, p2p_item_work=None):
         prm = _Params()
         lib().fmmh_default_params(ct.byref(prm))
         prm.p, prm.eps_mac, prm.theta_max, prm.n_leaf = int(p), float(eps), float(theta_max), int(n_leaf)
@@ -198,8 +205,25 @@ class Fmm:
             raise FmmError(rc, "invalid parameters")
         self._h = h
         self.n = None
+        if m2l_chunk is not None or p2p_item_work is not None:
+            self.set_tuning(m2l_chunk=m2l_chunk, p2p_item_work=p2p_item_work)
         if pos is not None:
             self.refresh(pos, mass)
+
+    def tuning(self):
+        """Tile shapes of this context: {"m2l_chunk", "p2p_item_work"}."""
+        t = _Tuning()
+        self._check(lib().fmmh_get_tuning(self._h, ct.byref(t)))
+        return {"m2l_chunk": int(t.m2l_chunk), "p2p_item_work": int(t.p2p_item_work)}
+
+    def set_tuning(self, m2l_chunk=None, p2p_item_work=None):
+        """Set tile shapes (None keeps the current value); they take effect at
+        the next full evaluation."""
+        cur = self.tuning()
+        t = _Tuning()
+        t.m2l_chunk = int(cur["m2l_chunk"] if m2l_chunk is None else m2l_chunk)
+        t.p2p_item_work = int(cur["p2p_item_work"] if p2p_item_work is None else p2p_item_work)
+        self._check(lib().fmmh_set_tuning(self._h, ct.byref(t)))
 
     @classmethod
     def with_center(cls, pos, mass, p, eps, n_leaf, mode, **kw):

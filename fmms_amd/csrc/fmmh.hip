@@ -32,11 +32,14 @@ enum Stage {
   ST_M2L_REDUCE,
   ST_L2L,
   ST_L2P,
-  ST_P2P,
+  ST_P2P_PREP,  // source ranges of the directed entries
+  ST_P2P,       // the pair kernel alone
+  ST_P2P_REDUCE,
   ST_COUNT
 };
-const char* kStageNames[ST_COUNT] = {"tree", "upward",     "prepass", "walk", "lists",
-                                     "m2l",  "m2l_reduce", "l2l",     "l2p",  "p2p"};
+const char* kStageNames[ST_COUNT] = {"tree", "upward", "prepass",  "walk", "lists",
+                                     "m2l",  "m2l_reduce", "l2l", "l2p", "p2p_prep",
+                                     "p2p",  "p2p_reduce"};
 
 struct Buf {
   void* p = nullptr;
@@ -62,6 +65,7 @@ struct Budget {
 
 struct fmmh_ctx {
   fmmh_params prm;
+  fmmh_tuning tun = {32, (uint64_t)1 << 18};
   hipStream_t s = nullptr;
   std::string err;
   size_t used = 0;
@@ -412,7 +416,9 @@ struct fmmh_ctx {
   uint32_t tree_evals = 0;  // evaluations run on the current tree and lists
   bool capturing = false;
 
-  // Gather (optional) + P2M + M2M into `moments` (already sized).
+  // Gather (optional) + P2M + M2M into `moments` (already sized); in the
+  // mixed form with the theta MAC, FP32 scaled moments below the FP64 top
+  // levels go straight into m2l_m32.
   void upward(const double* pos, const double* mass, bool regather) {
     const int p = prm.p;
     const size_t D = (size_t)hdim(p);
@@ -421,7 +427,19 @@ struct fmmh_ctx {
     if (regather) launch_gather(pos, mass, order, n, b, s);
     mark(ST_UPWARD);
     cplx* mom = (cplx*)moments.p;
-    if (p <= TEMPLATED_MAX_P) {
+    if (up32()) {
+      double* lam = ensure<double>(m2l_lam, (size_t)nc * sizeof(double));
+      void* m32 = ensure(m2l_m32, (size_t)nc * m2l32_moment_bytes(p));
+      const int ltop = vlev + FP64_TOP;
+      launch_m2l_scale(nc, c, (const BBox*)bbox.p, keys_sorted, vlev, lam, s);
+      launch_p2m32(p, (uint32_t*)leaf_bc.p, nleaves, c, b, lam, m32, s);
+      for (int L = (int)max_level - 1; L >= ltop; L--)
+        launch_m2m32(p, (uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, lam,
+                     m32, s);
+      launch_m32_to_f64(p, nc, c, ltop, lam, m32, mom, s);  // children of the top levels
+      for (int L = std::min(ltop, (int)max_level) - 1; L >= 0; L--)
+        launch_m2m(p, (uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, mom, s);
+    } else if (p <= TEMPLATED_MAX_P) {
       launch_p2m(p, (uint32_t*)leaf_bc.p, nleaves, c, b, mom, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
         launch_m2m(p, (uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, mom, s);
@@ -437,7 +455,8 @@ struct fmmh_ctx {
       launch_trace_leaf((uint32_t*)leaf_ids.p, nleaves, c, b, q, s);
       for (int L = (int)max_level - 1; L >= 0; L--)
         launch_trace_up((uint32_t*)int_cells.p + int_off[L], int_off[L + 1] - int_off[L], c, p, mom,
-                        q, s);
+                        q, s, up32() ? m2l_m32.p : nullptr, (const double*)m2l_lam.p,
+                        vlev + FP64_TOP);
     }
   }
 
@@ -499,13 +518,14 @@ struct fmmh_ctx {
                      (unsigned long long*)p2p_work.p, s);
     plan_p2p();
     if (p <= TEMPLATED_MAX_P) {
-      size_t maxck = m2l_max_chunks(2 * n_m2l, nc);
+      const uint32_t ch = (uint32_t)tun.m2l_chunk;
+      size_t maxck = m2l_max_chunks(2 * n_m2l, nc, ch);
       ensure(nch, ((size_t)nc + 1) * 4);
       ensure(choff, ((size_t)nc + 1) * 4);
       ensure(chunks, maxck * m2l_chunk_bytes());
       ensure_scan(nc + 1);
       uint32_t* tot = nullptr;
-      m2l_build_chunks((uint32_t*)tptr.p, nc, (uint32_t*)nch.p, (uint32_t*)choff.p,
+      m2l_build_chunks((uint32_t*)tptr.p, nc, ch, (uint32_t*)nch.p, (uint32_t*)choff.p,
                        (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
       // choff[nc] = total chunks (the reduce reads choff[t+1])
       HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
@@ -535,8 +555,8 @@ struct fmmh_ctx {
     uint32_t* nsplit = npart + w;
     const uint32_t* lids = (const uint32_t*)leaf_ids.p;
     const uint32_t* ccount = (const uint32_t*)c_count.p;
-    p2p_plan_count(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount, nitem, npart,
-                   nsplit, s);
+    p2p_plan_count(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount,
+                   tun.p2p_item_work, nitem, npart, nsplit, s);
     ensure_scan(w);
     uint32_t* tot = nullptr;
     scan_exclusive_u32(nitem, nitem, w, (uint32_t*)scan_tmp.p, s, &tot);
@@ -548,12 +568,15 @@ struct fmmh_ctx {
     ensure(pl_items, (size_t)std::max<uint32_t>(p_nitems, 1) * sizeof(uint4));
     ensure(pl_reds, (size_t)std::max<uint32_t>(p_nsplit, 1) * sizeof(uint4));
     ensure(pl_partial, (size_t)std::max<uint32_t>(nslots, 1) * sizeof(double4));
-    p2p_plan_emit(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount, nitem, npart,
-                  nsplit, (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
+    p2p_plan_emit(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount,
+                  tun.p2p_item_work, nitem, npart, nsplit, (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
   }
 
   // The mixed form's FP32 M2L, L2L and L2P (templated orders only).
   bool mixed32() const { return prm.precision == FMMH_PREC_FP32BULK && prm.p <= TEMPLATED_MAX_P; }
+  // ... and its FP32 P2M / M2M below the top levels (theta MAC: the error MAC
+  // reads the FP64 moments of every cell).
+  bool up32() const { return mixed32() && !std::isfinite(prm.eps_mac); }
 
   // M2L (+ reduce), L2L, L2P and the fused P2P + scatter into acc / pot.
   // Launches only: safe to capture into a graph once the lists exist.
@@ -568,7 +591,8 @@ struct fmmh_ctx {
     mark(ST_M2L);
     if (!gen && prm.precision == FMMH_PREC_FP32BULK) {
       launch_m2l32(p, nc, chunks.p, nchunks, msorted, sbits, c, (const BBox*)bbox.p, keys_sorted,
-                   vlev, mom, soft2, (double*)m2l_lam.p, m2l_m32.p, part.p, s);
+                   vlev, mom, soft2, (double*)m2l_lam.p, m2l_m32.p, part.p, s,
+                   up32() ? vlev + FP64_TOP : 255);
       mark(ST_M2L_REDUCE);
       launch_m2l_reduce32(p, nc, (uint32_t*)choff.p, part.p, (const double*)m2l_lam.p, loc, s);
     } else if (!gen) {
@@ -606,16 +630,20 @@ struct fmmh_ctx {
       launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, lap, (double4*)far.p, s);
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
-    mark(ST_P2P);
+    mark(ST_P2P_PREP);
     launch_p2p_ranges(psorted, pnnz, sbits, c, (uint2*)prange.p, s);
     const P2PPlan pl{(const uint4*)pl_items.p, p_nitems, (const uint4*)pl_reds.p, p_nsplit,
                      (double4*)pl_partial.p};
-    if (prm.precision == FMMH_PREC_FP32BULK)
+    const bool f32 = prm.precision == FMMH_PREC_FP32BULK;
+    mark(ST_P2P);
+    if (f32)
       launch_p2p_f32(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order,
                      acc, pot, s);
     else
       launch_p2p(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order, acc,
                  pot, s);
+    mark(ST_P2P_REDUCE);
+    launch_p2p_reduce(pl, c, (const double4*)far.p, order, acc, pot, f32, s);
     mark(ST_COUNT);
   }
 
@@ -1149,6 +1177,28 @@ int fmmh_evaluate_frozen(fmmh_ctx* ctx, size_t n, const double* pos, const doubl
   return rc;
 }
 
+void fmmh_default_tuning(fmmh_tuning* out) {
+  out->m2l_chunk = 32;
+  out->p2p_item_work = (uint64_t)1 << 18;
+}
+
+int fmmh_set_tuning(fmmh_ctx* ctx, const fmmh_tuning* t) {
+  if (!ctx) return FMMH_E_STATE;
+  if (!t || t->m2l_chunk < 1 || t->m2l_chunk > 1024 || t->p2p_item_work < 64 ||
+      t->p2p_item_work > ((uint64_t)1 << 40))
+    return fail(ctx, FMMH_E_PARAM, "tuning: m2l_chunk in [1, 1024], p2p_item_work in [64, 2^40]");
+  ctx->tun = *t;
+  ctx->lists_ok = false;  // the chunks and the P2P plan follow at the next full evaluation
+  ctx->drop_graph();
+  return FMMH_OK;
+}
+
+int fmmh_get_tuning(const fmmh_ctx* ctx, fmmh_tuning* out) {
+  if (!ctx || !out) return FMMH_E_STATE;
+  *out = ctx->tun;
+  return FMMH_OK;
+}
+
 size_t fmmh_num_cells(const fmmh_ctx* ctx) { return ctx && ctx->built ? ctx->nc : 0; }
 
 size_t fmmh_device_bytes(const fmmh_ctx* ctx) { return ctx ? ctx->used : 0; }
@@ -1218,6 +1268,16 @@ static int get_dev(fmmh_ctx* ctx, void* out, const void* src, size_t bytes) {
 
 int fmmh_get_moments(fmmh_ctx* ctx, double* out) {
   if (!ctx) return FMMH_E_STATE;
+  if (ctx->evaluated && ctx->up32()) {  // FP64 view of the scaled FP32 moments
+    int rc = guarded(ctx, [&] {
+      const size_t bytes = (size_t)ctx->nc * hdim(ctx->prm.p) * 16;
+      ctx->ensure(ctx->direct_out, bytes);
+      launch_m32_to_f64(ctx->prm.p, ctx->nc, ctx->cells(), -1, (const double*)ctx->m2l_lam.p,
+                        ctx->m2l_m32.p, (cplx*)ctx->direct_out.p, ctx->s);
+    });
+    if (rc) return rc;
+    return get_dev(ctx, out, ctx->direct_out.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
+  }
   return get_dev(ctx, out, ctx->moments.p, (size_t)ctx->nc * hdim(ctx->prm.p) * 16);
 }
 int fmmh_get_locals(fmmh_ctx* ctx, double* out) {

@@ -146,25 +146,47 @@ void launch_csr_ptr(const uint64_t* keys, uint32_t nnz, int sbits, uint32_t ntar
                     uint32_t* ptr, hipStream_t s);
 
 // --------------------------------------------------------------- m2l.hip
-size_t m2l_max_chunks(uint32_t nnz, uint32_t nc);
+// Chunks of at most `ch` sources per target (the M2L work unit).
+size_t m2l_max_chunks(uint32_t nnz, uint32_t nc, uint32_t ch);
 size_t m2l_part_bytes(int p, size_t maxchunks);
 size_t m2l_chunk_bytes();
-void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t* choff,
-                      uint32_t* scan_tmp, void* chunks, hipStream_t s, uint32_t** total_dev);
+void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t ch, uint32_t* nch,
+                      uint32_t* choff, uint32_t* scan_tmp, void* chunks, hipStream_t s,
+                      uint32_t** total_dev);
 bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                        int sbits, DevCells c, const cplx* moments, double soft2, cplx* part,
                        hipStream_t s);
 // Mixed form: the same M2L on radius-scaled FP32 operands (m2l_kernel.h);
 // lam [nc] doubles and m32 [nc] x m2l32_moment_bytes(p) are scratch.
+// prep_below: FP32 moments are made from the FP64 ones for the cells above
+// this level only (the cells below hold them from the FP32 upward pass);
+// 255 = every cell.
 bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                   int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
                   const cplx* moments, double soft2, double* lam, void* m32, void* part,
-                  hipStream_t s);
+                  hipStream_t s, int prep_below = 255);
+// Per-cell scale of the mixed form: radius, floored at a quarter of the octant edge.
+void launch_m2l_scale(uint32_t nc, DevCells c, const BBox* bb, const uint64_t* keys_sorted,
+                      int vlev, double* lam, hipStream_t s);
 size_t m2l32_moment_bytes(int p);
 bool launch_m2l_reduce32(int p, uint32_t nc, const uint32_t* choff, const void* part,
                          const double* lam, cplx* locals, hipStream_t s);
 bool launch_m2l_reduce(int p, uint32_t nc, const uint32_t* choff, const cplx* part, cplx* locals,
                        hipStream_t s);
+
+// ----------------------------------------------------------- upward32.hip
+// Mixed form's upward pass below the FP64 top levels: FP32 moments scaled by
+// lam^-n (m >= 0 half, m32 [nc] x hnpos(p) complex floats); and their FP64
+// view (sel >= 0: the cells at level sel and the leaves above it; sel < 0:
+// every cell).
+bool launch_p2m32(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
+                  const double* lam, void* m32, hipStream_t s);
+bool launch_m2m32(int p, const uint32_t* cells, uint32_t cnt, DevCells c, const double* lam,
+                  void* m32, hipStream_t s);
+bool launch_m32_to_f64(int p, uint32_t nc, DevCells c, int sel, const double* lam,
+                       const void* m32, cplx* out, hipStream_t s);
+// Levels below a system root whose M2M stays FP64 in the mixed form.
+constexpr int FP64_TOP = 3;
 
 // ------------------------------------------- generic_up.hip / generic_far.hip
 constexpr int TEMPLATED_MAX_P = 10;  // above this the runtime-p kernels run
@@ -205,8 +227,12 @@ bool launch_loc32_export(int p, uint32_t nc, const double* lam, const void* loc3
 // Trace terms of the softened far field (lap may be null: eps = 0).
 void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
                        double* q, hipStream_t s);
+// The children's dipoles come from mom (FP64), or, for children deeper than
+// level m32_below, from the mixed form's scaled FP32 moments m32 (with lam)
+// when m32 is non-null.
 void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, const cplx* mom,
-                     double* q, hipStream_t s);
+                     double* q, hipStream_t s, const void* m32 = nullptr,
+                     const double* lam = nullptr, int m32_below = 255);
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,
                       const double4* cpm, const double* q, double soft2, int p, cplx* locals,
                       double* lap, hipStream_t s);
@@ -253,13 +279,14 @@ struct P2PPlan {
   uint32_t nsplit;
   double4* partial;
 };
+// item_work: a leaf whose targets x sources exceed it is cut into items.
 void p2p_plan_count(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint32_t* nitem,
-                    uint32_t* npart, uint32_t* nsplit, hipStream_t s);
+                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint64_t item_work,
+                    uint32_t* nitem, uint32_t* npart, uint32_t* nsplit, hipStream_t s);
 void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                   const uint64_t* skeys, int sbits, const uint32_t* ccount, const uint32_t* ioff,
-                   const uint32_t* poff, const uint32_t* roff, uint4* items, uint4* reds,
-                   hipStream_t s);
+                   const uint64_t* skeys, int sbits, const uint32_t* ccount, uint64_t item_work,
+                   const uint32_t* ioff, const uint32_t* poff, const uint32_t* roff, uint4* items,
+                   uint4* reds, hipStream_t s);
 void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
                        hipStream_t s);
 void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
@@ -268,6 +295,10 @@ void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2*
 void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
                     double soft2, const double4* far, const uint32_t* order, double* accel,
                     double* pot, hipStream_t s);
+// Leaves cut into several items: their partial sums and the far field, added
+// in item order (compensated in the mixed form) into accel / pot.
+void launch_p2p_reduce(const P2PPlan& pl, DevCells c, const double4* far, const uint32_t* order,
+                       double* accel, double* pot, bool compensated, hipStream_t s);
 // Directed body-body interactions of a P2P pair list (self pairs c(c-1));
 // out[0] = total, out needs 1 + p2p_count_scratch(np) words.
 size_t p2p_count_scratch(uint32_t np);

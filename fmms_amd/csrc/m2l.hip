@@ -1,4 +1,4 @@
-// m2l.hip — M2L work decomposition (chunks of <= M2L_CH sources per target),
+// m2l.hip — M2L work decomposition (chunks of at most m2l_chunk sources per target),
 // the per-target reduction of chunk partials and the order dispatch. The
 // contraction kernels live in m2l_kernel.h, instantiated per range of orders
 // by m2l_p*.hip (separate translation units so they build in parallel).
@@ -33,14 +33,16 @@ __global__ void __launch_bounds__(256) k_m2l_scale(uint32_t nc, DevCells c,
   lam[i] = l;
 }
 
-// Moments of every cell scaled by lam^-k, m >= 0 half, FP32.
+// Moments scaled by lam^-k, m >= 0 half, FP32, of the cells above level
+// `below` (the others come from the FP32 upward pass).
 template <int P>
 __global__ void __launch_bounds__(256) k_m2l_prep32(uint32_t nc, const cplx* __restrict__ moments,
                                                    const double* __restrict__ lam,
+                                                   const uint8_t* __restrict__ level, int below,
                                                    cplxf* __restrict__ m32) {
   constexpr int D = hdim(P), NF = hnpos(P);
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nc) return;
+  if (i >= nc || (int)level[i] >= below) return;
   const cplx* ms = moments + (size_t)i * D;
   cplxf* o = m32 + (size_t)i * NF;
   const double il = 1.0 / lam[i];
@@ -57,21 +59,21 @@ __global__ void __launch_bounds__(256) k_m2l_prep32(uint32_t nc, const cplx* __r
 }
 
 __global__ void __launch_bounds__(256) k_m2l_nchunks(const uint32_t* __restrict__ tptr, uint32_t nc,
-                                                    uint32_t* __restrict__ nch) {
+                                                    uint32_t ch, uint32_t* __restrict__ nch) {
   uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= nc) return;
   uint32_t s = tptr[t + 1] - tptr[t];
-  nch[t] = (s + M2L_CH - 1) / M2L_CH;
+  nch[t] = (s + ch - 1) / ch;
 }
 
 __global__ void __launch_bounds__(256) k_m2l_emit(const uint32_t* __restrict__ tptr, uint32_t nc,
-                                                 const uint32_t* __restrict__ choff,
+                                                 uint32_t ch, const uint32_t* __restrict__ choff,
                                                  M2lChunk* __restrict__ chunks) {
   uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= nc) return;
   uint32_t b = tptr[t], e = tptr[t + 1];
   uint32_t w = choff[t];
-  for (uint32_t j = b; j < e; j += M2L_CH) chunks[w++] = M2lChunk{t, j, min(e, j + M2L_CH)};
+  for (uint32_t j = b; j < e; j += ch) chunks[w++] = M2lChunk{t, j, min(e, j + ch)};
 }
 
 constexpr int RED_AHEAD = 8;  // chunk partials loaded ahead of the in-order sum
@@ -163,7 +165,7 @@ __global__ void __launch_bounds__(256) k_m2l_reduce32(uint32_t nc, const uint32_
 
 // Scratch layout handed in by the context: nch/choff (nc+1 words each) and
 // chunk/partial arrays sized for the worst case sum_t ceil(S_t/CH).
-size_t m2l_max_chunks(uint32_t nnz, uint32_t nc) { return (size_t)nnz / M2L_CH + nc + 1; }
+size_t m2l_max_chunks(uint32_t nnz, uint32_t nc, uint32_t ch) { return (size_t)nnz / ch + nc + 1; }
 size_t m2l_part_bytes(int p, size_t maxchunks) { return maxchunks * hnpos(p) * sizeof(cplx); }
 
 template <int P>
@@ -174,11 +176,12 @@ static void m2l_reduce_t(uint32_t nc, const uint32_t* choff, const cplx* part, c
                      locals);
 }
 
-void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t* nch, uint32_t* choff,
-                      uint32_t* scan_tmp, void* chunks, hipStream_t s, uint32_t** total_dev) {
-  hipLaunchKernelGGL(k_m2l_nchunks, dim3((nc + 255) / 256), dim3(256), 0, s, tptr, nc, nch);
+void m2l_build_chunks(const uint32_t* tptr, uint32_t nc, uint32_t ch, uint32_t* nch,
+                      uint32_t* choff, uint32_t* scan_tmp, void* chunks, hipStream_t s,
+                      uint32_t** total_dev) {
+  hipLaunchKernelGGL(k_m2l_nchunks, dim3((nc + 255) / 256), dim3(256), 0, s, tptr, nc, ch, nch);
   scan_exclusive_u32(nch, choff, (size_t)nc, scan_tmp, s, total_dev);
-  hipLaunchKernelGGL(k_m2l_emit, dim3((nc + 255) / 256), dim3(256), 0, s, tptr, nc, choff,
+  hipLaunchKernelGGL(k_m2l_emit, dim3((nc + 255) / 256), dim3(256), 0, s, tptr, nc, ch, choff,
                      (M2lChunk*)chunks);
 }
 
@@ -208,18 +211,24 @@ bool launch_m2l_kernel(int p, const void* chunks, uint32_t nchunks, const uint64
   return true;
 }
 
+void launch_m2l_scale(uint32_t nc, DevCells c, const BBox* bb, const uint64_t* keys_sorted,
+                      int vlev, double* lam, hipStream_t s) {
+  if (nc == 0) return;
+  hipLaunchKernelGGL(k_m2l_scale, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, bb, keys_sorted,
+                     vlev, lam);
+}
+
 bool launch_m2l32(int p, uint32_t nc, const void* chunks, uint32_t nchunks, const uint64_t* skeys,
                   int sbits, DevCells c, const BBox* bb, const uint64_t* keys_sorted, int vlev,
                   const cplx* moments, double soft2, double* lam, void* m32, void* part,
-                  hipStream_t s) {
+                  hipStream_t s, int prep_below) {
   const uint64_t smask = (1ull << sbits) - 1ull;
   const M2lChunk* ck = (const M2lChunk*)chunks;
   if (nc == 0) return true;
-  hipLaunchKernelGGL(k_m2l_scale, dim3((nc + 255) / 256), dim3(256), 0, s, nc, c, bb, keys_sorted,
-                     vlev, lam);
+  launch_m2l_scale(nc, c, bb, keys_sorted, vlev, lam, s);
 #define CALL(P)                                                                            \
   hipLaunchKernelGGL(k_m2l_prep32<P>, dim3((nc + 255) / 256), dim3(256), 0, s, nc, moments, \
-                     (const double*)lam, (cplxf*)m32);                                     \
+                     (const double*)lam, c.level, prep_below, (cplxf*)m32);                \
   m2l32_t<P>(ck, nchunks, skeys, smask, c, lam, (const cplxf*)m32, soft2, (cplxf*)part, s)
   FMMH_P_SWITCH3(p, CALL)
 #undef CALL

@@ -6,7 +6,7 @@
 // grouped by target (sorted (target, source) keys), so every local is
 // written by exactly one reduction, in a fixed order, with no atomics.
 //
-// Mapping: one thread per chunk of <= M2L_CH sources of one target. Per
+// Mapping: one thread per chunk of at most m2l_chunk sources of one target. Per
 // source the thread streams Theta degree by degree (three-term recursion,
 // two previous columns in registers) against the source multipole held in
 // registers, accumulating the m >= 0 half of F (the m < 0 half follows from
@@ -19,10 +19,8 @@
 
 namespace fmmh {
 
-#ifndef FMMH_M2L_CH
-#define FMMH_M2L_CH 32
-#endif
-constexpr int M2L_CH = FMMH_M2L_CH;  // sources per chunk (one thread's serial work)
+// The chunk length (sources per thread) is a context parameter,
+// fmmh_tuning.m2l_chunk (default 32).
 
 __device__ __forceinline__ void cmac(cplx& f, double ar, double ai, double br, double bi) {
   f.re = fma(ar, br, f.re);

@@ -335,18 +335,15 @@ __global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__
 // A leaf's P2P work is its target count times its source bodies (own leaf
 // plus neighbour leaves). One block per leaf leaves the whole device waiting
 // on the few leaves whose neighbourhoods are huge (a large sparse leaf beside
-// a dense region), so a leaf whose work exceeds P2P_ITEM_WORK is cut into
-// m = ceil(work / P2P_ITEM_WORK) work items (at most one per source unit).
+// a dense region), so a leaf whose work exceeds the context's item bound W
+// (fmmh_tuning.p2p_item_work, default 2^18) is cut into m = ceil(work / W)
+// work items (at most one per source unit).
 // Source units are the own leaf (unit 0) and the neighbour ranges in list
 // order; unit u with P_u sources before it goes to item
 // min(m - 1, P_u / ceil(S / m)) (items may come out empty). Split items write
 // partial sums that k_p2p_reduce adds in item order. The cut depends on the
 // leaf's own counts only, so a system evaluates the same alone and in an
 // ensemble. One wave per leaf in both plan kernels.
-#ifndef FMMH_P2P_ITEM_WORK
-#define FMMH_P2P_ITEM_WORK (1ull << 18)
-#endif
-constexpr uint64_t P2P_ITEM_WORK = FMMH_P2P_ITEM_WORK;
 constexpr int PLAN_T = 256;
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
@@ -364,7 +361,8 @@ struct LeafCut {
 __device__ __forceinline__ LeafCut leaf_cut(uint32_t i, const uint32_t* __restrict__ leaf_ids,
                                             const uint32_t* __restrict__ pptr,
                                             const uint64_t* __restrict__ skeys, uint64_t smask,
-                                            const uint32_t* __restrict__ ccount, uint32_t lane) {
+                                            const uint32_t* __restrict__ ccount, uint64_t W,
+                                            uint32_t lane) {
   LeafCut L;
   L.leaf = leaf_ids[i];
   L.nt = ccount[L.leaf];
@@ -377,7 +375,7 @@ __device__ __forceinline__ LeafCut leaf_cut(uint32_t i, const uint32_t* __restri
   }
   L.S = (uint64_t)L.nt + wave_sum_u64(s);
   const uint64_t work = (uint64_t)L.nt * L.S;
-  uint64_t m = work > P2P_ITEM_WORK ? (work + P2P_ITEM_WORK - 1) / P2P_ITEM_WORK : 1;
+  uint64_t m = work > W ? (work + W - 1) / W : 1;
   const uint64_t units = (uint64_t)(L.je - L.jb) + 1;
   L.m = m > units ? units : m;
   return L;
@@ -389,7 +387,7 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_count(const uint32_t* __res
                                                           const uint64_t* __restrict__ skeys,
                                                           uint64_t smask,
                                                           const uint32_t* __restrict__ ccount,
-                                                          uint32_t* __restrict__ nitem,
+                                                          uint64_t W, uint32_t* __restrict__ nitem,
                                                           uint32_t* __restrict__ npart,
                                                           uint32_t* __restrict__ nsplit) {
   const uint32_t i = blockIdx.x * (PLAN_T / 64) + (threadIdx.x >> 6);
@@ -399,7 +397,7 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_count(const uint32_t* __res
     if (lane == 0) nitem[i] = npart[i] = nsplit[i] = 0u;
     return;
   }
-  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, lane);
+  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, W, lane);
   if (lane == 0) {
     const uint32_t m = (uint32_t)L.m;
     nitem[i] = m;
@@ -414,6 +412,7 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_emit(const uint32_t* __rest
                                                          const uint64_t* __restrict__ skeys,
                                                          uint64_t smask,
                                                          const uint32_t* __restrict__ ccount,
+                                                         uint64_t W,
                                                          const uint32_t* __restrict__ ioff,
                                                          const uint32_t* __restrict__ poff,
                                                          const uint32_t* __restrict__ roff,
@@ -422,7 +421,7 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_emit(const uint32_t* __rest
   const uint32_t i = blockIdx.x * (PLAN_T / 64) + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   if (i >= nleaves) return;
-  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, lane);
+  const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, W, lane);
   const uint32_t m = (uint32_t)L.m;
   const uint32_t base = ioff[i], pb = poff[i];
   uint32_t* it = (uint32_t*)(items + base);  // [m][4]: leaf|own, begin, end, slot
@@ -516,24 +515,24 @@ __global__ void __launch_bounds__(64) k_p2p_reduce(const uint4* __restrict__ red
 // Work plan of the current P2P lists. nitem / npart / nsplit are nleaves + 1
 // words each (scanned in place); the totals come back through the host.
 void p2p_plan_count(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint32_t* nitem,
-                    uint32_t* npart, uint32_t* nsplit, hipStream_t s) {
+                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint64_t item_work,
+                    uint32_t* nitem, uint32_t* npart, uint32_t* nsplit, hipStream_t s) {
   if (nleaves == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
   hipLaunchKernelGGL(k_p2p_plan_count, dim3((nleaves + 1 + PLAN_T / 64 - 1) / (PLAN_T / 64)),
                      dim3(PLAN_T), 0, s, leaf_ids,
-                     nleaves, pptr, skeys, smask, ccount, nitem, npart, nsplit);
+                     nleaves, pptr, skeys, smask, ccount, item_work, nitem, npart, nsplit);
 }
 
 void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* pptr,
-                   const uint64_t* skeys, int sbits, const uint32_t* ccount, const uint32_t* ioff,
-                   const uint32_t* poff, const uint32_t* roff, uint4* items, uint4* reds,
-                   hipStream_t s) {
+                   const uint64_t* skeys, int sbits, const uint32_t* ccount, uint64_t item_work,
+                   const uint32_t* ioff, const uint32_t* poff, const uint32_t* roff, uint4* items,
+                   uint4* reds, hipStream_t s) {
   if (nleaves == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
   hipLaunchKernelGGL(k_p2p_plan_emit, dim3((nleaves + PLAN_T / 64 - 1) / (PLAN_T / 64)),
                      dim3(PLAN_T), 0, s, leaf_ids,
-                     nleaves, pptr, skeys, smask, ccount, ioff, poff, roff, items, reds);
+                     nleaves, pptr, skeys, smask, ccount, item_work, ioff, poff, roff, items, reds);
 }
 
 void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
@@ -550,9 +549,17 @@ void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const ui
   if (pl.nitems == 0) return;
   hipLaunchKernelGGL(k_p2p_f32, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
-  if (pl.nsplit)
-    hipLaunchKernelGGL(k_p2p_reduce<true>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
-                       pl.partial, far, order, accel, pot);
+}
+
+void launch_p2p_reduce(const P2PPlan& pl, DevCells c, const double4* far, const uint32_t* order,
+                       double* accel, double* pot, bool compensated, hipStream_t s) {
+  if (pl.nitems == 0 || pl.nsplit == 0) return;
+  if (compensated)
+    hipLaunchKernelGGL(k_p2p_reduce<true>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first,
+                       c.count, pl.partial, far, order, accel, pot);
+  else
+    hipLaunchKernelGGL(k_p2p_reduce<false>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first,
+                       c.count, pl.partial, far, order, accel, pot);
 }
 
 void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
@@ -561,9 +568,6 @@ void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2*
   if (pl.nitems == 0) return;
   hipLaunchKernelGGL(k_p2p, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first, c.count,
                      body, soft2, far, order, accel, pot, pl.partial);
-  if (pl.nsplit)
-    hipLaunchKernelGGL(k_p2p_reduce<false>, dim3(pl.nsplit), dim3(64), 0, s, pl.reds, c.first, c.count,
-                       pl.partial, far, order, accel, pot);
 }
 
 __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,

@@ -57,6 +57,8 @@ __global__ void __launch_bounds__(ST) k_trace_leaf(const uint32_t* __restrict__ 
 // (M_1^0 = sum m dz, M_1^1 = sum m (dx + i dy) / 2).
 __global__ void __launch_bounds__(ST) k_trace_up(const uint32_t* __restrict__ cells, uint32_t cnt,
                                                 DevCells c, int D, const cplx* __restrict__ mom,
+                                                int NF, const cplxf* __restrict__ m32,
+                                                const double* __restrict__ lam, int m32_below,
                                                 double* __restrict__ q) {
   const uint32_t i = blockIdx.x * ST + threadIdx.x;
   if (i >= cnt) return;
@@ -68,8 +70,22 @@ __global__ void __launch_bounds__(ST) k_trace_up(const uint32_t* __restrict__ ce
     const int ch = c.children[(size_t)id * 8 + o];
     if (ch < 0) continue;
     const double dx = c.cx[ch] - zx, dy = c.cy[ch] - zy, dz = c.cz[ch] - zz;
-    const cplx* mc = mom + (size_t)ch * D;
-    const double px = 2.0 * mc[3].re, py = 2.0 * mc[3].im, pz = mc[2].re;
+    double px, py, pz;
+    // cells below level m32_below hold only scaled FP32 moments (the mixed
+    // form's FP32 upward pass): M_1 = Mt_1 lam (m >= 0 layout: (1,0) at 1,
+    // (1,1) at 2); the others hold FP64 moments
+    if (m32 && (int)c.level[ch] > m32_below) {
+      const cplxf* mc = m32 + (size_t)ch * NF;
+      const double l = lam[ch];
+      px = 2.0 * (double)mc[2].re * l;
+      py = 2.0 * (double)mc[2].im * l;
+      pz = (double)mc[1].re * l;
+    } else {
+      const cplx* mc = mom + (size_t)ch * D;
+      px = 2.0 * mc[3].re;
+      py = 2.0 * mc[3].im;
+      pz = mc[2].re;
+    }
     acc += q[ch] + 2.0 * (px * dx + py * dy + pz * dz) + c.mass[ch] * (dx * dx + dy * dy + dz * dz);
   }
   q[id] = acc;
@@ -177,10 +193,11 @@ void launch_trace_leaf(const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, D
 }
 
 void launch_trace_up(const uint32_t* cells, uint32_t cnt, DevCells c, int p, const cplx* mom,
-                     double* q, hipStream_t s) {
+                     double* q, hipStream_t s, const void* m32, const double* lam,
+                     int m32_below) {
   if (cnt == 0) return;
   hipLaunchKernelGGL(k_trace_up, dim3((cnt + ST - 1) / ST), dim3(ST), 0, s, cells, cnt, c, hdim(p),
-                     mom, q);
+                     mom, hnpos(p), (const cplxf*)m32, lam, m32_below, q);
 }
 
 void launch_m2l_trace(uint32_t nc, const uint32_t* tptr, const uint64_t* skeys, int sbits,

@@ -102,6 +102,22 @@ int fmmh_evaluate_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const dou
 int fmmh_accel_device(fmmh_ctx* ctx, size_t n, const double* d_pos, const double* d_mass,
                       double* d_accel, double* d_pot);
 
+/* Tile shapes of the evaluation, settable per context (no rebuild needed;
+ * they take effect at the next full evaluation). They replace the reference
+ * GPU app's compile-time constants: its P2P split of leaves with many partner
+ * bodies (FAT_THR / FAT_CHUNK, crates/visual/src/gpufmm.rs:686-689) and its
+ * fixed preset (gpufmm.rs:55-58, whose p and n_leaf are fmmh_params here).
+ * fmms_amd.autotune picks them (with p and n_leaf) by measurement. */
+typedef struct fmmh_tuning {
+  int m2l_chunk;          /* sources per M2L work chunk (one thread's serial work), 1..1024;
+                             default 32 */
+  uint64_t p2p_item_work; /* a leaf whose targets x sources exceed this is cut into P2P work
+                             items of about equal source count, 64..2^40; default 2^18 */
+} fmmh_tuning;
+void fmmh_default_tuning(fmmh_tuning* out);
+int fmmh_set_tuning(fmmh_ctx* ctx, const fmmh_tuning* t);
+int fmmh_get_tuning(const fmmh_ctx* ctx, fmmh_tuning* out);
+
 /* Device bytes this context holds (its arena of stage buffers); callers that
  * admit work under a memory budget read it after a call. No reference
  * counterpart (the reference's GPU app sizes buffers up front,

@@ -67,3 +67,27 @@ def test_sample_ids_matches_oracle():
     import fmms_amd
 
     assert np.array_equal(fmms_amd.sample_ids(100_000, 300, 7), O.sample_ids(100_000, 300, 7))
+
+
+def test_default_tuning_is_host_only():
+    import fmms_amd
+
+    t = fmms_amd._Tuning()
+    fmms_amd.lib().fmmh_default_tuning(ct.byref(t))
+    assert t.m2l_chunk == 32 and t.p2p_item_work == 1 << 18
+
+
+def test_autotune_cache_roundtrip(tmp_path):
+    from fmms_amd import autotune
+
+    k1 = autotune.workload_key(1 << 20, "fp32bulk", 0.5, 1e-3, p_grid=(8,), leaf_grid=(64, 128))
+    k2 = autotune.workload_key(1 << 20, "fp64", 0.5, 1e-3, p_grid=(8,), leaf_grid=(64, 128))
+    k3 = autotune.workload_key(1 << 20, "fp32bulk", 0.5, 1e-4, p_grid=(8,), leaf_grid=(64, 128))
+    assert len({k1, k2, k3}) == 3
+    cache = str(tmp_path / "tune.json")
+    assert autotune.load(k1, cache) is None
+    choice = {"p": 8, "n_leaf": 128, "m2l_chunk": 32, "p2p_item_work": 1 << 18, "ms": 1.0}
+    autotune._store(k1, choice, cache)
+    autotune._store(k2, dict(choice, n_leaf=96), cache)
+    assert autotune.load(k1, cache) == choice
+    assert autotune.load(k2, cache)["n_leaf"] == 96

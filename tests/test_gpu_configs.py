@@ -326,3 +326,53 @@ def test_energy_gate_full(fa):
     mom = O.momentum_rel(mass, it.vel.download())
     say(f"  drift {drift:.3e} max_dev {max_dev:.3e} slope {slope:.3e}/step momentum {mom:.3e}")
     assert abs(drift) < 1e-3 and max_dev < 1e-3 and abs(slope) < 1e-6 and mom < 1e-12
+
+
+def test_tuning_parameters(fa):
+    """Tile shapes change the work decomposition, not the approximation:
+    lists identical, forces within rounding of the defaults; invalid values
+    are rejected; the autotuner returns a choice that meets the bar and stores
+    it."""
+    import tempfile
+
+    from fmms_amd import autotune
+
+    pos, mass = O.plummer(30000, seed=12)
+    kw = dict(p=8, eps=math.inf, n_leaf=64, theta_max=0.5, softening=1e-3)
+    say("[tuning] tiles and autotune")
+    base = fa.Fmm(None, None, **kw)
+    r0 = base.accel(pos, mass)
+    m0, p0 = base.lists()
+    assert base.tuning() == {"m2l_chunk": 32, "p2p_item_work": 1 << 18}
+    for ch, iw in ((8, 1 << 12), (64, 1 << 22), (1, 64)):
+        g = fa.Fmm(None, None, m2l_chunk=ch, p2p_item_work=iw, **kw)
+        r = g.accel(pos, mass)
+        m1, p1 = g.lists()
+        assert sorted(map(tuple, m1)) == sorted(map(tuple, m0))
+        assert sorted(map(tuple, p1)) == sorted(map(tuple, p0))
+        check_forces(r.accel, r.pot, r0.accel, r0.pot, rel=1e-13)
+        if iw <= 1 << 12:
+            assert g.work_counts()["p2p_split_leaves"] > base.work_counts()["p2p_split_leaves"]
+    for bad in (dict(m2l_chunk=0), dict(m2l_chunk=2000), dict(p2p_item_work=10)):
+        with pytest.raises(fa.FmmError) as e:
+            base.set_tuning(**bad)
+        assert e.value.code == fa.FMMH_E_PARAM
+    with tempfile.TemporaryDirectory() as d:
+        cache = d + "/tune.json"
+        bar = 5e-4  # met at this N by some of the grid (the trials show which)
+        res = autotune.tune(pos, mass, p_grid=(4, 8), leaf_grid=(32, 64), chunk_grid=(16, 32),
+                            item_grid=(1 << 16, 1 << 18), precision="fp32bulk", theta=0.5,
+                            softening=1e-3, max_rel_err=bar, samples=512, repeats=1, cache=cache)
+        say(f"  choice {res['p']} {res['n_leaf']} {res['m2l_chunk']} {res['p2p_item_work']} "
+            f"p99 {res['rel_p99']:.2e}; trials {[(t['p'], t['n_leaf'], round(t['ms'], 2), t['rel_p99']) for t in res['trials']]}")
+        valid = [t for t in res["trials"] if t["rel_p99"] <= bar]
+        assert valid and res["valid"] and res["rel_p99"] <= bar
+        fastest = min(valid, key=lambda t: t["ms"])
+        assert (res["p"], res["n_leaf"]) == (fastest["p"], fastest["n_leaf"])
+        assert res["ms"] == min(t["ms"] for t in res["tile_trials"])
+        key = autotune.workload_key(len(mass), "fp32bulk", 0.5, 1e-3, p_grid=(4, 8),
+                                    leaf_grid=(32, 64), max_rel_err=bar)
+        assert autotune.load(key, cache)["n_leaf"] == res["n_leaf"]
+        g = autotune.make_fmm(res, theta=0.5, softening=1e-3)
+        assert g.tuning() == {"m2l_chunk": res["m2l_chunk"], "p2p_item_work": res["p2p_item_work"]}
+        g.accel(pos, mass)
