@@ -37,7 +37,7 @@ SOURCES = {
     "prims.hip": "off",
     "tree.hip": "off",
     "upward.hip": "off",
-    "upward32.hip": "fast",
+    "upward32.hip": "off",
     "walk.hip": "off",
     "m2l.hip": "fast",
     "m2l_p1_6.hip": "fast",

@@ -111,6 +111,7 @@ struct fmmh_ctx {
   Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
+  Buf walk_code, walk_blk;  // the walk's per-pair decisions, per-block counts
   Buf pl_n, pl_items, pl_reds, pl_partial;  // P2P work plan
   Buf m2l_lam, m2l_m32;                     // mixed form: FP32 M2L scales, scaled moments
   Buf loc32;                                // mixed form: FP32 locals scaled by lam^n
@@ -132,7 +133,7 @@ struct fmmh_ctx {
                   &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &pl_n, &pl_items,
+                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &walk_code, &walk_blk, &pl_n, &pl_items,
                   &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &loc32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &direct_part, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
@@ -495,9 +496,12 @@ struct fmmh_ctx {
     ensure(mkeys_alt, (size_t)2 * n_m2l * 8);
     ensure(tptr, ((size_t)nc + 1) * 4);
     ensure(sort_tmp, radix_tmp_words((size_t)2 * n_m2l) * 4 + 64);
+    // The walk writes its pairs in one fixed order, so a stable sort on the
+    // target bits alone groups the directed entries deterministically (each
+    // target's sources in walk order).
     launch_directed_keys_m2l((uint2*)m2l.p, n_m2l, sbits, (uint64_t*)mkeys.p, s);
     int fl = radix_sort_u64((uint64_t*)mkeys.p, nullptr, (uint64_t*)mkeys_alt.p, nullptr,
-                            (size_t)2 * n_m2l, 0, kbits, (uint32_t*)sort_tmp.p, s);
+                            (size_t)2 * n_m2l, sbits, kbits, (uint32_t*)sort_tmp.p, s);
     msorted = (const uint64_t*)(fl ? mkeys_alt.p : mkeys.p);
     launch_csr_ptr(msorted, 2 * n_m2l, sbits, nc, (uint32_t*)tptr.p, s);
     ensure(pkeys, (size_t)2 * n_p2p * 8);
@@ -508,7 +512,7 @@ struct fmmh_ctx {
     HCHECK(hipMemsetAsync(pcount.p, 0, 4, s));
     launch_directed_keys_p2p((uint2*)p2p.p, n_p2p, sbits, (uint64_t*)pkeys.p, (uint32_t*)pcount.p, s);
     fl = radix_sort_u64((uint64_t*)pkeys.p, nullptr, (uint64_t*)pkeys_alt.p, nullptr,
-                        (size_t)2 * n_p2p, 0, kbits, (uint32_t*)sort_tmp.p, s);
+                        (size_t)2 * n_p2p, sbits, kbits, (uint32_t*)sort_tmp.p, s);
     psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
     uint32_t nself = read1((const uint32_t*)pcount.p);
     pnnz = 2 * n_p2p - nself;
@@ -823,11 +827,15 @@ struct fmmh_ctx {
     rot_p = p;
   }
 
+  // The dual walk, breadth-first (walk.hip): per round the decisions and
+  // their per-block counts, then the outputs in frontier order; one readback
+  // per round gives the next frontier's size. An output buffer that ran out
+  // of room grows and the round's outputs are written again.
   void walk(bool use_err) {
-    const uint32_t cap0 = 4 * nc + 1024;
-    if (front_a.cap < (size_t)cap0 * 8) {
-      ensure(front_a, (size_t)cap0 * 8);
-      ensure(front_b, (size_t)cap0 * 8);
+    const size_t cap0 = (size_t)4 * nc + 1024;
+    if (front_a.cap < cap0 * 8) {
+      ensure(front_a, cap0 * 8);
+      ensure(front_b, cap0 * 8);
     }
     ensure(m2l, (size_t)(32 * (size_t)nc + 1024) * 8);
     ensure(p2p, (size_t)(16 * (size_t)nc + 1024) * 8);
@@ -845,31 +853,33 @@ struct fmmh_ctx {
     // the walk starts from the self pair of every (system) root: level vlev
     uint32_t nf = lvl_off[vlev + 1] - lvl_off[vlev];
     launch_front_init((const uint32_t*)lvl_cells.p + lvl_off[vlev], nf, front_a.p, s);
+    WalkResult* res = (WalkResult*)ctr.p;
+    HCHECK(hipMemsetAsync(res, 0, sizeof(WalkResult), s));
     n_m2l = n_p2p = 0;
     walk_rounds = 0;
     Buf* fin = &front_a;
     Buf* fout = &front_b;
-    WalkCounters* hc = (WalkCounters*)host_stage(sizeof(WalkCounters));
+    auto cap = [](const Buf& b) { return (uint32_t)std::min<size_t>(b.cap / 8, 0xFFFFFFFFu); };
     while (nf > 0) {
-      WalkCounters init{0, n_m2l, n_p2p, 0};
-      *hc = init;
-      HCHECK(hipMemcpyAsync(ctr.p, hc, sizeof init, hipMemcpyHostToDevice, s));
-      launch_walk_round((const uint2*)fin->p, nf, (uint2*)fout->p, (uint32_t)(fout->cap / 8),
-                        (uint2*)m2l.p, (uint32_t)(m2l.cap / 8), (uint2*)p2p.p,
-                        (uint32_t)(p2p.cap / 8), (WalkCounters*)ctr.p, walkrec.p,
-                        (const double*)c_mass.p, mp, s);
-      HCHECK(hipMemcpyAsync(hc, ctr.p, sizeof init, hipMemcpyDeviceToHost, s));
-      HCHECK(hipStreamSynchronize(s));
-      WalkCounters got = *hc;
-      if (got.overflow) {
-        if (got.next > fout->cap / 8) ensure(*fout, (size_t)got.next * 8 * 2);
-        if (got.m2l > m2l.cap / 8) ensure(m2l, (size_t)got.m2l * 8 * 2, true);
-        if (got.p2p > p2p.cap / 8) ensure(p2p, (size_t)got.p2p * 8 * 2, true);
-        continue;  // re-run the round: its input frontier is unchanged
+      ensure(walk_code, (size_t)nf * 2);
+      ensure(walk_blk, (size_t)walk_blocks(nf) * 12);
+      launch_walk_decide((const uint2*)fin->p, nf, (uint16_t*)walk_code.p, (uint32_t*)walk_blk.p,
+                         walkrec.p, (const double*)c_mass.p, mp, res, s);
+      WalkResult r;
+      for (;;) {
+        launch_walk_emit((const uint2*)fin->p, nf, (uint2*)fout->p, cap(*fout), (uint2*)m2l.p,
+                         cap(m2l), (uint2*)p2p.p, cap(p2p), (const uint16_t*)walk_code.p,
+                         (const uint32_t*)walk_blk.p, n_m2l, n_p2p, res, walkrec.p, s);
+        r = read1((const WalkResult*)res);
+        if (r.status == 0) break;
+        if (r.status & 1u) ensure(*fout, (size_t)r.next * 8 * 2);
+        if (r.status & 2u) ensure(m2l, (size_t)r.n_m2l * 8 * 2, true);
+        if (r.status & 4u) ensure(p2p, (size_t)r.n_p2p * 8 * 2, true);
+        HCHECK(hipMemsetAsync(&res->status, 0, sizeof(uint32_t), s));
       }
-      n_m2l = got.m2l;
-      n_p2p = got.p2p;
-      nf = got.next;
+      n_m2l = r.n_m2l;
+      n_p2p = r.n_p2p;
+      nf = r.next;
       std::swap(fin, fout);
       walk_rounds++;
       if (fin->cap > fout->cap) ensure(*fout, fin->cap);

@@ -113,12 +113,6 @@ void launch_fmin(const uint32_t* cells, uint32_t cnt, DevCells c, const double* 
                  hipStream_t s);
 
 // -------------------------------------------------------------- walk.hip
-struct WalkCounters {
-  uint32_t next;
-  uint32_t m2l;
-  uint32_t p2p;
-  uint32_t overflow;
-};
 struct MacParams {
   int p;
   int use_err;
@@ -134,9 +128,23 @@ size_t walk_rec_bytes();
 void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s);
 // Walk records, and {cx, cy, cz, mass} of every cell as one 32-byte record.
 void launch_pack_walk(uint32_t nc, DevCells c, void* rec, double4* cpm, hipStream_t s);
-void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
-                       uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
-                       const void* rec, const double* mass, MacParams mac, hipStream_t s);
+// One round of the walk (walk.hip) in frontier order: decide + per-block
+// counts + their scan (res->next = the next frontier's size, res->n_m2l /
+// n_p2p advanced by the round's pairs), then the outputs (status bits 1 / 2
+// / 4 = frontier / M2L / P2P capacity exceeded: nothing of that kind was
+// written, run the emit again with more room). blk holds 3 x walk_blocks(nf)
+// words; code nf.
+struct WalkResult {
+  uint32_t next, n_m2l, n_p2p, status;
+};
+uint32_t walk_blocks(uint32_t nf);
+void launch_walk_decide(const uint2* fin, uint32_t nf, uint16_t* code, uint32_t* blk,
+                        const void* rec, const double* mass, MacParams mac, WalkResult* res,
+                        hipStream_t s);
+void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
+                      uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, const uint16_t* code,
+                      const uint32_t* blk, uint32_t m2l0, uint32_t p2p0, WalkResult* res,
+                      const void* rec, hipStream_t s);
 void launch_directed_keys_m2l(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
                               hipStream_t s);
 // returns via ctr->next the number of directed P2P entries written

@@ -8,16 +8,19 @@
 //                         kappa^(n-k) Mt_{n-k}^{m-l}(child),  kappa = lam_c / lam_p <= 1.
 // The cells within FP64_TOP levels of a system root keep FP64 moments: their
 // children's scaled moments are converted back (M = Mt lam^n) and the FP64
-// M2M of upward.hip runs there.
+// M2M of upward.hip runs there. Compiled without multiply-add contraction
+// (the sums use explicit fmaf), so the two P2M mappings below give the same
+// bits and a system's moments do not depend on the size of its ensemble.
 #include "harm32.h"
 #include "internal.h"
 
 namespace fmmh {
 
-// One thread per leaf (leaves taken in body-count order, so a wave's lanes
-// loop over nearly equal numbers of bodies).
+// P2M with one thread per leaf (leaves taken in body-count order, so a
+// wave's lanes loop over nearly equal numbers of bodies): each body is read
+// once; for trees with enough leaves to fill the device.
 template <int P>
-__global__ void __launch_bounds__(64) k_p2m32(const uint32_t* __restrict__ leaf_ids,
+__global__ void __launch_bounds__(64) k_p2m32_leaf(const uint32_t* __restrict__ leaf_ids,
                                              uint32_t nleaves, const double* __restrict__ cx,
                                              const double* __restrict__ cy,
                                              const double* __restrict__ cz,
@@ -63,7 +66,73 @@ __global__ void __launch_bounds__(64) k_p2m32(const uint32_t* __restrict__ leaf_
   for (int e2 = 0; e2 < NF; e2++) o[e2] = acc[e2];
 }
 
-// One thread per internal cell of a level (its children's levels done).
+// P2M with one thread per (leaf, order m), for trees with too few leaves to
+// fill the device with one thread per leaf: thread t takes column m = t / L of leaf
+// t % L (L = leaves rounded up to whole waves), so a wave's lanes share m and
+// run the same recursion, over leaves taken in body-count order (nearly
+// equal trip counts). Column m: Y_m^m by the diagonal chain, then
+// Y_n^m = ((2n-1) z Y_{n-1}^m - r^2 Y_{n-2}^m) / (n^2 - m^2) (harmonic.rs:141-173).
+template <int P>
+__global__ void __launch_bounds__(64) k_p2m32_col(const uint32_t* __restrict__ leaf_ids,
+                                             uint32_t nleaves, uint32_t lpad,
+                                             const double* __restrict__ cx,
+                                             const double* __restrict__ cy,
+                                             const double* __restrict__ cz,
+                                             const uint32_t* __restrict__ first,
+                                             const uint32_t* __restrict__ count,
+                                             const double4* __restrict__ xyzm,
+                                             const double* __restrict__ lam,
+                                             cplxf* __restrict__ m32) {
+  constexpr int NF = hnpos(P);
+  const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t li = t % lpad;
+  const int m = (int)(t / lpad);
+  if (li >= nleaves || m > P) return;
+  const uint32_t id = leaf_ids[li];
+  const double zx = cx[id], zy = cy[id], zz = cz[id], il = 1.0 / lam[id];
+  cplxf acc[P + 1];
+#pragma unroll
+  for (int n = 0; n <= P; n++) acc[n] = cplxf{0.0f, 0.0f};
+  const uint32_t f = first[id], e = f + count[id];
+  for (uint32_t k = f; k < e; k++) {
+    const double4 q = xyzm[k];
+    const float x = (float)((q.x - zx) * il), y = (float)((q.y - zy) * il),
+                z = (float)((q.z - zz) * il);
+    const float r2 = x * x + y * y + z * z;
+    const float ms = (float)q.w;
+    cplxf u = cplxf{1.0f, 0.0f};  // Y_j^j up to j = m
+    for (int j = 1; j <= m; j++) {
+      const float g = 1.0f / (float)(2 * j);
+      const float wr = x * g, wi = y * g;
+      u = cplxf{wr * u.re - wi * u.im, wr * u.im + wi * u.re};
+    }
+    cplxf u1 = u, u2 = cplxf{0.0f, 0.0f};
+#pragma unroll
+    for (int n = 0; n <= P; n++) {
+      if (n < m) continue;
+      cplxf un;
+      if (n == m) {
+        un = u;
+      } else {
+        const float a = (float)(2 * n - 1) * z;
+        const float ic = 1.0f / (float)(n * n - m * m);
+        un = cplxf{(a * u1.re - r2 * u2.re) * ic, (a * u1.im - r2 * u2.im) * ic};
+        u2 = u1;
+        u1 = un;
+      }
+      acc[n].re = fmaf(ms, un.re, acc[n].re);
+      acc[n].im = fmaf(ms, un.im, acc[n].im);
+    }
+  }
+  cplxf* o = m32 + (size_t)id * NF;
+#pragma unroll
+  for (int n = 0; n <= P; n++)
+    if (n >= m) o[hpos(n, m)] = acc[n];
+}
+
+// One lane per (internal cell of a level, octant): eight lanes form one
+// parent; each computes its child's shifted moments, and the eight are added
+// by a fixed xor-shuffle tree (children's levels done before).
 template <int P>
 __global__ void __launch_bounds__(64) k_m2m32(const uint32_t* __restrict__ cells, uint32_t cnt,
                                              const int32_t* __restrict__ children,
@@ -73,18 +142,18 @@ __global__ void __launch_bounds__(64) k_m2m32(const uint32_t* __restrict__ cells
                                              const double* __restrict__ lam,
                                              cplxf* __restrict__ m32) {
   constexpr int NF = hnpos(P);
-  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= cnt) return;
-  const uint32_t id = cells[i];
-  const double zx = cx[id], zy = cy[id], zz = cz[id], il = 1.0 / lam[id];
+  const uint32_t g = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t pi = g >> 3, o = g & 7u;
+  const bool valid = pi < cnt;
+  const uint32_t id = valid ? cells[pi] : 0u;
+  const int ch = valid ? children[(size_t)id * 8 + o] : -1;
   cplxf acc[NF];
 #pragma unroll
   for (int e = 0; e < NF; e++) acc[e] = cplxf{0.0f, 0.0f};
-  for (int o = 0; o < 8; o++) {
-    const int ch = children[(size_t)id * 8 + o];
-    if (ch < 0) continue;
-    const float x = (float)((cx[ch] - zx) * il), y = (float)((cy[ch] - zy) * il),
-                z = (float)((cz[ch] - zz) * il);
+  if (ch >= 0) {
+    const double il = 1.0 / lam[id];
+    const float x = (float)((cx[ch] - cx[id]) * il), y = (float)((cy[ch] - cy[id]) * il),
+                z = (float)((cz[ch] - cz[id]) * il);
     const float r2 = x * x + y * y + z * z;
     const float kap = (float)(lam[ch] * il);
     cplxf M[NF];
@@ -127,9 +196,18 @@ __global__ void __launch_bounds__(64) k_m2m32(const uint32_t* __restrict__ cells
       }
     }
   }
-  cplxf* out = m32 + (size_t)id * NF;
 #pragma unroll
-  for (int e = 0; e < NF; e++) out[e] = acc[e];
+  for (int sft = 1; sft < 8; sft <<= 1)
+#pragma unroll
+    for (int e = 0; e < NF; e++) {
+      acc[e].re += __shfl_xor(acc[e].re, sft, 64);
+      acc[e].im += __shfl_xor(acc[e].im, sft, 64);
+    }
+  if (valid && o == 0) {
+    cplxf* out = m32 + (size_t)id * NF;
+#pragma unroll
+    for (int e = 0; e < NF; e++) out[e] = acc[e];
+  }
 }
 
 // FP64 moments M = Mt lam^n (both halves of m) of the cells that sel picks:
@@ -179,9 +257,16 @@ __global__ void __launch_bounds__(256) k_m32_to_f64(uint32_t nc, const uint8_t* 
 bool launch_p2m32(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c, DevBodies b,
                   const double* lam, void* m32, hipStream_t s) {
   if (nleaves == 0) return true;
-#define CALL(P)                                                                               \
-  hipLaunchKernelGGL(k_p2m32<P>, dim3((nleaves + 63) / 64), dim3(64), 0, s, leaf_ids, nleaves, \
-                     c.cx, c.cy, c.cz, c.first, c.count, b.xyzm, lam, (cplxf*)m32)
+  // one thread per leaf once there are enough leaves for 1024 waves
+  const uint32_t lpad = (nleaves + 63) / 64 * 64;
+  const bool per_leaf = nleaves >= 65536;
+#define CALL(P)                                                                                    \
+  if (per_leaf)                                                                                    \
+    hipLaunchKernelGGL(k_p2m32_leaf<P>, dim3((nleaves + 63) / 64), dim3(64), 0, s, leaf_ids,       \
+                       nleaves, c.cx, c.cy, c.cz, c.first, c.count, b.xyzm, lam, (cplxf*)m32);     \
+  else                                                                                             \
+    hipLaunchKernelGGL(k_p2m32_col<P>, dim3(lpad / 64 * (P + 1)), dim3(64), 0, s, leaf_ids,        \
+                       nleaves, lpad, c.cx, c.cy, c.cz, c.first, c.count, b.xyzm, lam, (cplxf*)m32)
   FMMH_P_SWITCH4(p, CALL)
 #undef CALL
   return true;
@@ -190,8 +275,8 @@ bool launch_p2m32(int p, const uint32_t* leaf_ids, uint32_t nleaves, DevCells c,
 bool launch_m2m32(int p, const uint32_t* cells, uint32_t cnt, DevCells c, const double* lam,
                   void* m32, hipStream_t s) {
   if (cnt == 0) return true;
-#define CALL(P)                                                                                \
-  hipLaunchKernelGGL(k_m2m32<P>, dim3((cnt + 63) / 64), dim3(64), 0, s, cells, cnt, c.children, \
+#define CALL(P)                                                                                 \
+  hipLaunchKernelGGL(k_m2m32<P>, dim3((cnt + 7) / 8), dim3(64), 0, s, cells, cnt, c.children,    \
                      c.cx, c.cy, c.cz, lam, (cplxf*)m32)
   FMMH_P_SWITCH4(p, CALL)
 #undef CALL

@@ -5,10 +5,12 @@
 // (i <= j); both MAC accepts -> mutual M2L (a,b); two leaves -> P2P (a,b);
 // otherwise split the leaf's partner, or the larger-rho cell (ties split a).
 // The pair set this produces does not depend on traversal order, so the GPU
-// walks it breadth-first: one thread per frontier pair per round, children
-// appended to the next frontier, M2L / P2P pairs appended to their lists with
-// wave-aggregated atomics. The MAC (mac.rs:23-95) is evaluated in the
+// walks it breadth-first, one thread per frontier pair per round: children
+// are written to the next frontier and M2L / P2P pairs to their lists at
+// offsets from scans of the per-block counts, in frontier order (so the lists
+// come out in one fixed order). The MAC (mac.rs:23-95) is evaluated in the
 // reference's operation order, so decisions agree bit for bit.
+
 #include "common.h"
 #include "internal.h"
 
@@ -108,128 +110,198 @@ __device__ __forceinline__ uint32_t nchild(const WalkRec& r) {
   return k;
 }
 
-constexpr int WALK_T = 1024;
+// walk.rs:94-190 decision of one frontier pair; nch = its child pairs.
+__device__ __forceinline__ int decide(uint32_t a, uint32_t b, const WalkRec& ra,
+                                      const WalkRec& rb, const double* __restrict__ mass,
+                                      const MacParams& mp, uint32_t* nch) {
+  const uint32_t na = nchild(ra), nb = nchild(rb);
+  const bool la = na == 0, lb = nb == 0;
+  *nch = 0;
+  if (a == b) {
+    if (la) return A_P2P;
+    *nch = na * (na + 1) / 2;
+    return A_SELF_SPLIT;
+  }
+  double dx = rb.cx - ra.cx, dy = rb.cy - ra.cy, dz = rb.cz - ra.cz;
+  double r2ab = dx * dx + dy * dy + dz * dz;
+  double ex = ra.cx - rb.cx, ey = ra.cy - rb.cy, ez = ra.cz - rb.cz;
+  double r2ba = ex * ex + ey * ey + ez * ez;
+  if (accept(a, ra, b, rb, r2ab, mass, mp) && accept(b, rb, a, ra, r2ba, mass, mp)) return A_M2L;
+  if (la && lb) return A_P2P;
+  bool split_b;
+  if (la)
+    split_b = true;
+  else if (lb)
+    split_b = false;
+  else
+    split_b = !(ra.rho >= rb.rho);
+  *nch = split_b ? nb : na;
+  return split_b ? A_SPLIT_B : A_SPLIT_A;
+}
 
-__global__ void __launch_bounds__(WALK_T) k_walk_round(const uint2* __restrict__ fin, uint32_t nf,
-                                                   uint2* __restrict__ fout, uint32_t cap_f,
-                                                   uint2* __restrict__ m2l, uint32_t cap_m2l,
-                                                   uint2* __restrict__ p2p, uint32_t cap_p2p,
-                                                   WalkCounters* __restrict__ ctr,
-                                                   const WalkRec* __restrict__ rec,
-                                                   const double* __restrict__ mass, MacParams mp) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = t < nf;
-  uint32_t a = 0, b = 0;
-  int act = A_NONE;
-  uint32_t nch = 0;
-  WalkRec ra, rb;
-  if (valid) {
-    uint2 pr = fin[t];
-    a = pr.x;
-    b = pr.y;
-    ra = rec[a];
-    rb = rec[b];
-    const uint32_t na = nchild(ra), nb = nchild(rb);
-    const bool la = na == 0, lb = nb == 0;
-    if (a == b) {
-      if (la) {
-        act = A_P2P;
-      } else {
-        act = A_SELF_SPLIT;
-        nch = na * (na + 1) / 2;
-      }
-    } else {
-      // walk.rs:138-139: accept(a,b) && accept(b,a); mac.rs:75-78 forms
-      // r = cen_b - cen_a for the first call and cen_a - cen_b for the second.
-      double dx = rb.cx - ra.cx, dy = rb.cy - ra.cy, dz = rb.cz - ra.cz;
-      double r2ab = dx * dx + dy * dy + dz * dz;
-      double ex = ra.cx - rb.cx, ey = ra.cy - rb.cy, ez = ra.cz - rb.cz;
-      double r2ba = ex * ex + ey * ey + ez * ez;
-      if (accept(a, ra, b, rb, r2ab, mass, mp) && accept(b, rb, a, ra, r2ba, mass, mp)) {
-        act = A_M2L;
-      } else if (la && lb) {
-        act = A_P2P;
-      } else {
-        bool split_b;
-        if (la)
-          split_b = true;
-        else if (lb)
-          split_b = false;
-        else
-          split_b = !(ra.rho >= rb.rho);
-        act = split_b ? A_SPLIT_B : A_SPLIT_A;
-        nch = split_b ? nb : na;
-      }
+constexpr int CW_T = 256;
+
+// Exclusive block scan of three counters (CW_T threads); totals in tot[3].
+__device__ __forceinline__ void block_scan3(const uint32_t v[3], uint32_t x[3], uint32_t tot[3],
+                                            uint32_t (*sh)[3]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t wt[3];
+  for (int k = 0; k < 3; k++) x[k] = wave_excl_scan(v[k], &wt[k]);
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) sh[w][k] = wt[k];
+  __syncthreads();
+  for (int k = 0; k < 3; k++) {
+    uint32_t off = 0, t = 0;
+    for (int q = 0; q < CW_T / 64; q++) {
+      if (q < w) off += sh[q][k];
+      t += sh[q][k];
     }
-  }
-  // --- appends, aggregated per block: one atomic per counter per block
-  __shared__ uint32_t wsum[WALK_T / 64][3];
-  __shared__ uint32_t wbase[WALK_T / 64][3];
-  const int w = threadIdx.x >> 6;
-  const uint64_t lt = lanemask_lt();
-  const uint64_t bm = __ballot(act == A_M2L);
-  const uint64_t bp = __ballot(act == A_P2P);
-  uint32_t wtot;
-  const uint32_t pre = wave_excl_scan(nch, &wtot);
-  if (lane == 0) {
-    wsum[w][0] = wtot;
-    wsum[w][1] = (uint32_t)__popcll(bm);
-    wsum[w][2] = (uint32_t)__popcll(bp);
+    x[k] += off;
+    tot[k] = t;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
-    const int k = threadIdx.x;
-    uint32_t tot = 0;
-    for (int q = 0; q < WALK_T / 64; q++) {
-      wbase[q][k] = tot;
-      tot += wsum[q][k];
-    }
-    uint32_t* ctrk = k == 0 ? &ctr->next : (k == 1 ? &ctr->m2l : &ctr->p2p);
-    const uint32_t b0 = tot ? atomicAdd(ctrk, tot) : 0u;
-    for (int q = 0; q < WALK_T / 64; q++) wbase[q][k] += b0;
+}
+
+// One round of the breadth-first walk in three launches, with outputs in
+// frontier order: (1) every frontier pair is decided, its decision stored
+// and its block's output counts written; (2) one block scans the per-block
+// counts into offsets and round totals; (3) every pair writes its outputs
+// (child pairs to the next frontier, M2L and P2P pairs appended to the
+// lists) at its block's offset plus its rank in the block. The lists thus
+// come out in one fixed order, and a system's pairs keep their relative
+// order whether it is walked alone or in an ensemble.
+__global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ fin, uint32_t nf,
+                                                    uint16_t* __restrict__ code,
+                                                    uint32_t* __restrict__ blk,
+                                                    const WalkRec* __restrict__ rec,
+                                                    const double* __restrict__ mass, MacParams mp) {
+  __shared__ uint32_t sh[CW_T / 64][3];
+  const uint32_t t = blockIdx.x * CW_T + threadIdx.x;
+  uint32_t v[3] = {0u, 0u, 0u};  // child pairs, M2L, P2P
+  if (t < nf) {
+    const uint2 pr = fin[t];
+    const WalkRec ra = rec[pr.x], rb = rec[pr.y];
+    uint32_t nch;
+    const int act = decide(pr.x, pr.y, ra, rb, mass, mp, &nch);
+    code[t] = (uint16_t)(act | (nch << 3));
+    v[0] = nch;
+    v[1] = act == A_M2L ? 1u : 0u;
+    v[2] = act == A_P2P ? 1u : 0u;
   }
-  __syncthreads();
-  if (nch) {
-    uint32_t wp = wbase[w][0] + pre;
-    if (wp + nch > cap_f) {
-      atomicOr(&ctr->overflow, 1u);
-    } else if (act == A_SELF_SPLIT) {
-      for (int i = 0; i < 8; i++) {
-        if (ra.ch[i] < 0) continue;
-        for (int j = i; j < 8; j++)
-          if (ra.ch[j] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
+  uint32_t x[3], tot[3];
+  block_scan3(v, x, tot, sh);
+  if (threadIdx.x < 3) blk[(size_t)gridDim.x * threadIdx.x + blockIdx.x] = tot[threadIdx.x];
+}
+
+// One block: exclusive scans of the three per-block count columns (nb each,
+// in place); res gets the round totals added to the running list lengths.
+__global__ void __launch_bounds__(1024) k_walk_scan(uint32_t* __restrict__ blk, uint32_t nb,
+                                                  WalkResult* __restrict__ res) {
+  __shared__ uint32_t sh[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < 3; k++) {
+    uint32_t* col = blk + (size_t)nb * k;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+      const uint32_t j = b0 + threadIdx.x;
+      const uint32_t v = j < nb ? col[j] : 0u;
+      uint32_t wt;
+      uint32_t x = wave_excl_scan(v, &wt);
+      if (lane == 0) sh[w] = wt;
+      __syncthreads();
+      uint32_t off = 0, t = 0;
+      for (int q = 0; q < 16; q++) {
+        if (q < w) off += sh[q];
+        t += sh[q];
       }
-    } else if (act == A_SPLIT_B) {
-      for (int j = 0; j < 8; j++)
-        if (rb.ch[j] >= 0) fout[wp++] = make_uint2(a, (uint32_t)rb.ch[j]);
-    } else {
-      for (int i = 0; i < 8; i++)
-        if (ra.ch[i] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], b);
+      if (j < nb) col[j] = carry + off + x;
+      carry += t;
+      __syncthreads();
     }
-  }
-  if (act == A_M2L) {
-    const uint32_t wp = wbase[w][1] + (uint32_t)__popcll(bm & lt);
-    if (wp < cap_m2l)
-      m2l[wp] = make_uint2(a, b);
-    else
-      atomicOr(&ctr->overflow, 2u);
-  }
-  if (act == A_P2P) {
-    const uint32_t wp = wbase[w][2] + (uint32_t)__popcll(bp & lt);
-    if (wp < cap_p2p)
-      p2p[wp] = make_uint2(a, b);
-    else
-      atomicOr(&ctr->overflow, 4u);
+    if (threadIdx.x == 0) {
+      if (k == 0) res->next = carry;
+      if (k == 1) res->n_m2l += carry;
+      if (k == 2) res->n_p2p += carry;
+    }
   }
 }
 
-void launch_walk_round(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
-                       uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, WalkCounters* ctr,
-                       const void* rec, const double* mass, MacParams mac, hipStream_t s) {
-  hipLaunchKernelGGL(k_walk_round, dim3((nf + WALK_T - 1) / WALK_T), dim3(WALK_T), 0, s, fin, nf,
-                     fout, cap_f,
-                     m2l, cap_m2l, p2p, cap_p2p, ctr, (const WalkRec*)rec, mass, mac);
+__global__ void __launch_bounds__(CW_T) k_walk_emit(const uint2* __restrict__ fin, uint32_t nf,
+                                                  uint2* __restrict__ fout, uint32_t cap_f,
+                                                  uint2* __restrict__ m2l, uint32_t cap_m2l,
+                                                  uint2* __restrict__ p2p, uint32_t cap_p2p,
+                                                  const uint16_t* __restrict__ code,
+                                                  const uint32_t* __restrict__ blk,
+                                                  uint32_t m2l0, uint32_t p2p0,
+                                                  WalkResult* __restrict__ res,
+                                                  const WalkRec* __restrict__ rec) {
+  __shared__ uint32_t sh[CW_T / 64][3];
+  const uint32_t t = blockIdx.x * CW_T + threadIdx.x;
+  const bool valid = t < nf;
+  const uint32_t cd = valid ? (uint32_t)code[t] : 0u;
+  const int act = valid ? (int)(cd & 7u) : A_NONE;
+  const uint32_t v[3] = {cd >> 3, act == A_M2L ? 1u : 0u, act == A_P2P ? 1u : 0u};
+  uint32_t x[3], tot[3];
+  block_scan3(v, x, tot, sh);
+  if (!valid || act == A_NONE) return;
+  const size_t nb = gridDim.x;
+  const uint2 pr = fin[t];
+  if (act == A_M2L) {
+    const uint32_t wp = m2l0 + blk[nb + blockIdx.x] + x[1];
+    if (wp < cap_m2l)
+      m2l[wp] = pr;
+    else
+      atomicOr(&res->status, 2u);
+    return;
+  }
+  if (act == A_P2P) {
+    const uint32_t wp = p2p0 + blk[2 * nb + blockIdx.x] + x[2];
+    if (wp < cap_p2p)
+      p2p[wp] = pr;
+    else
+      atomicOr(&res->status, 4u);
+    return;
+  }
+  uint32_t wp = blk[blockIdx.x] + x[0];
+  if (wp + v[0] > cap_f) {
+    atomicOr(&res->status, 1u);
+    return;
+  }
+  if (act == A_SELF_SPLIT) {
+    const WalkRec ra = rec[pr.x];
+    for (int i = 0; i < 8; i++) {
+      if (ra.ch[i] < 0) continue;
+      for (int j = i; j < 8; j++)
+        if (ra.ch[j] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
+    }
+  } else if (act == A_SPLIT_B) {
+    const WalkRec rb = rec[pr.y];
+    for (int j = 0; j < 8; j++)
+      if (rb.ch[j] >= 0) fout[wp++] = make_uint2(pr.x, (uint32_t)rb.ch[j]);
+  } else {
+    const WalkRec ra = rec[pr.x];
+    for (int i = 0; i < 8; i++)
+      if (ra.ch[i] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], pr.y);
+  }
+}
+
+uint32_t walk_blocks(uint32_t nf) { return (nf + CW_T - 1) / CW_T; }
+
+void launch_walk_decide(const uint2* fin, uint32_t nf, uint16_t* code, uint32_t* blk,
+                        const void* rec, const double* mass, MacParams mac, WalkResult* res,
+                        hipStream_t s) {
+  const uint32_t nb = walk_blocks(nf);
+  hipLaunchKernelGGL(k_walk_decide, dim3(nb), dim3(CW_T), 0, s, fin, nf, code, blk,
+                     (const WalkRec*)rec, mass, mac);
+  hipLaunchKernelGGL(k_walk_scan, dim3(1), dim3(1024), 0, s, blk, nb, res);
+}
+
+void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
+                      uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, const uint16_t* code,
+                      const uint32_t* blk, uint32_t m2l0, uint32_t p2p0, WalkResult* res,
+                      const void* rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_walk_emit, dim3(walk_blocks(nf)), dim3(CW_T), 0, s, fin, nf, fout, cap_f,
+                     m2l, cap_m2l, p2p, cap_p2p, code, blk, m2l0, p2p0, res, (const WalkRec*)rec);
 }
 
 // Directed M2L entries (target << sbits | source), both directions per pair.
