@@ -782,9 +782,10 @@ def main():
         dst.free()
 
     form_names = {"fp64": "plain FP64 GPU drop-in",
-                  "fp32bulk": ("mixed: P2P pair terms in FP32 on leaf-centred positions and M2L in "
-                               "FP32 on radius-scaled operands, FP64 tile / chunk sums and final "
-                               "accumulation; tree, upward pass, L2L and L2P in FP64")}
+                  "fp32bulk": ("mixed: P2P pair terms in FP32 on leaf-centred positions; P2M, "
+                               "M2M below the top three levels, M2L, L2L and L2P in FP32 on "
+                               "radius-scaled operands; FP64 positions, top-level M2M, tile and "
+                               "chunk sums, compensated final per-body sum")}
     out = {
         "metric": METRIC,
         "value": best["value"],

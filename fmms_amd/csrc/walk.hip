@@ -194,35 +194,49 @@ __global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ 
 }
 
 // One block: exclusive scans of the three per-block count columns (nb each,
-// in place); res gets the round totals added to the running list lengths.
+// in place), 4096 entries of each column per pass (four per thread); res
+// gets the round totals added to the running list lengths.
 __global__ void __launch_bounds__(1024) k_walk_scan(uint32_t* __restrict__ blk, uint32_t nb,
                                                   WalkResult* __restrict__ res) {
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[16][3];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int k = 0; k < 3; k++) {
-    uint32_t* col = blk + (size_t)nb * k;
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
-      const uint32_t j = b0 + threadIdx.x;
-      const uint32_t v = j < nb ? col[j] : 0u;
-      uint32_t wt;
-      uint32_t x = wave_excl_scan(v, &wt);
-      if (lane == 0) sh[w] = wt;
-      __syncthreads();
+  uint32_t carry[3] = {0u, 0u, 0u};
+  for (uint32_t b0 = 0; b0 < nb; b0 += 4096) {
+    const uint32_t j0 = b0 + 4 * threadIdx.x;
+    uint32_t v[3][4], sum[3];
+    for (int k = 0; k < 3; k++) {
+      const uint32_t* col = blk + (size_t)nb * k;
+      sum[k] = 0;
+      for (int q = 0; q < 4; q++) {
+        v[k][q] = j0 + q < nb ? col[j0 + q] : 0u;
+        sum[k] += v[k][q];
+      }
+    }
+    uint32_t x[3], wt[3];
+    for (int k = 0; k < 3; k++) x[k] = wave_excl_scan(sum[k], &wt[k]);
+    if (lane == 0)
+      for (int k = 0; k < 3; k++) sh[w][k] = wt[k];
+    __syncthreads();
+    for (int k = 0; k < 3; k++) {
       uint32_t off = 0, t = 0;
       for (int q = 0; q < 16; q++) {
-        if (q < w) off += sh[q];
-        t += sh[q];
+        if (q < w) off += sh[q][k];
+        t += sh[q][k];
       }
-      if (j < nb) col[j] = carry + off + x;
-      carry += t;
-      __syncthreads();
+      uint32_t* col = blk + (size_t)nb * k;
+      uint32_t run = carry[k] + off + x[k];
+      for (int q = 0; q < 4; q++) {
+        if (j0 + q < nb) col[j0 + q] = run;
+        run += v[k][q];
+      }
+      carry[k] += t;
     }
-    if (threadIdx.x == 0) {
-      if (k == 0) res->next = carry;
-      if (k == 1) res->n_m2l += carry;
-      if (k == 2) res->n_p2p += carry;
-    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res->next = carry[0];
+    res->n_m2l += carry[1];
+    res->n_p2p += carry[2];
   }
 }
 
