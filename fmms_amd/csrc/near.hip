@@ -1,48 +1,19 @@
-// near.hip — P2P near field on gfx950 (FP64), fused with the final
-// near + far sum and the slot -> particle-id scatter.
+// near.hip — P2P near field on gfx950 (FP64 and mixed forms), fused with the
+// final near + far sum and the slot -> particle-id scatter.
 //
 // Reference: crates/fmm p2p.rs:13-88 (mutual a_i += m_j d/r^3, a_j -= m_i
 // d/r^3, pot += m/r) and translate.rs:339-342 (accel[id] += near + far).
-// Here each target leaf owns its sums: one wave per target leaf walks the
-// leaf's directed source list (its own slots included, self excluded by
-// slot), staging each source leaf's bodies through LDS in tiles of 64. A
-// leaf of c <= 64 bodies uses 64/pow2ceil(c) lanes per target body, each on
-// a strided share of the source tile, combined by xor-shuffles at the end.
+// Here each target leaf owns its sums: one wave per work item (a leaf, or a
+// slice of a heavy leaf's neighbour list) streams the item's source bodies
+// (its own leaf included, self excluded by slot) through LDS in tiles of 64.
+// The leaf's targets run as power-of-two chunks of n bodies with 64/n lanes
+// per body, each lane on a strided share of every tile, combined by
+// xor-shuffles at the end.
 // Softening (an extension; 0 = the reference): r^2 += eps^2.
 #include "common.h"
 #include "internal.h"
 
 namespace fmmh {
-
-// Bodies of a 64-slot LDS tile against the lane's target: lane (ti, sub)
-// takes tile entries sub, sub+K, ... (K = 64 / pow2ceil(targets)). Entries
-// past the sources are padded with mass 0 at 1e100, which contributes 0.
-template <bool SELF>
-__device__ __forceinline__ void p2p_tile(const double4* __restrict__ tile, uint32_t sub,
-                                         uint32_t K, double xi, double yi, double zi, double soft2,
-                                         int self_j, double& ax, double& ay, double& az,
-                                         double& ph) {
-#pragma unroll 4
-  for (uint32_t j = sub; j < 64; j += K) {
-    const double4 sj = tile[j];
-    const double dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
-    double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft2)));
-    double mj = sj.w;
-    if (SELF) {
-      const bool self = (int)j == self_j;
-      r2 = self ? 1.0 : r2;
-      mj = self ? 0.0 : mj;
-    }
-    // r2 is finite and > 0 here (softening, or distinct bodies, or padding)
-    const double inv = rsqrt_pos(r2);
-    const double minv = mj * inv;
-    const double s3 = minv * inv * inv;
-    ax = fma(s3, dx, ax);
-    ay = fma(s3, dy, ay);
-    az = fma(s3, dz, az);
-    ph += minv;
-  }
-}
 
 constexpr uint32_t P2P_RMAX = 256;  // source ranges staged in LDS per pass
 
@@ -84,120 +55,130 @@ __device__ __forceinline__ TileLoad gather_tile(const uint2* rng, uint32_t nr, u
   return t;
 }
 
-__global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
-                                           const uint2* __restrict__ prange,
-                                           const uint32_t* __restrict__ cfirst,
-                                           const uint32_t* __restrict__ ccount,
-                                           const double4* __restrict__ body, double soft2,
-                                           const double4* __restrict__ far,
-                                           const uint32_t* __restrict__ order,
-                                           double* __restrict__ accel, double* __restrict__ pot,
-                                           double4* __restrict__ partial) {
-  __shared__ double4 tile[64];
-  __shared__ uint2 rng[P2P_RMAX];
-  const uint32_t lane = threadIdx.x;
-  const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
-  const uint32_t leaf = it.x & P2P_LEAF_MASK;
-  const bool own = (it.x & P2P_OWN) != 0u;
-  const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
-  const uint32_t jb = it.y, je = it.z;
-  const double4 pad = make_double4(1e100, 1e100, 1e100, 0.0);
-  for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
-    const uint32_t nt = min(64u, nt_all - t0);
-    uint32_t ntp = 1;
-    while (ntp < nt) ntp <<= 1;
-    const uint32_t K = 64 / ntp;
-    const uint32_t ti = lane & (ntp - 1);
-    const uint32_t sub = lane / ntp;
-    const bool tvalid = ti < nt;
-    const uint32_t si = ft + t0 + (tvalid ? ti : 0);
-    const double4 me = body[si];
-    const double xi = me.x, yi = me.y, zi = me.z;
-    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
-    // 1) the target's own leaf (first work item of the leaf), excluding each
-    // body's self term
-    for (uint32_t s0 = 0; own && s0 < nt_all; s0 += 64) {
-      __syncthreads();
-      const bool in = s0 + lane < nt_all;
-      double4 v = body[in ? ft + s0 + lane : ft];
-      if (!in) v = pad;
-      tile[lane] = v;
-      __syncthreads();
-      const int self_j = (int)(t0 + ti) - (int)s0;
-      p2p_tile<true>(tile, sub, K, xi, yi, zi, soft2, self_j, ax, ay, az, ph);
+// The two forms of the pairwise terms. FP64: absolute positions, the
+// Newton-refined rsqrt. Mixed: float4 positions relative to the target
+// leaf's centre (the subtraction done in FP64), the hardware FP32 rsqrt, and
+// each tile's FP32 partial sums added into FP64 accumulators.
+template <bool F32> struct P2PForm;
+template <> struct P2PForm<false> {
+  using T = double;
+  using T4 = double4;
+  static __device__ __forceinline__ double rsq(double x) { return rsqrt_pos(x); }
+  static __device__ __forceinline__ double fmat(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+  }
+  static __device__ __forceinline__ double4 stage(double4 v, double, double, double) { return v; }
+};
+template <> struct P2PForm<true> {
+  using T = float;
+  using T4 = float4;
+  static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+  static __device__ __forceinline__ float fmat(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+  }
+  static __device__ __forceinline__ float4 stage(double4 v, double cx, double cy, double cz) {
+    return make_float4((float)(v.x - cx), (float)(v.y - cy), (float)(v.z - cz), (float)v.w);
+  }
+};
+
+// Bodies of a 64-entry LDS tile against the lane's target: the lane takes
+// entries sub, sub + K, ... (ts = tile + sub), 64 / K of them. K is a
+// template argument, so every lane runs the same trip count (no per-lane
+// exit test) and the LDS reads take immediate offsets. Padding entries have
+// mass 0 far away and contribute exactly 0. SELF: the entry at offset
+// self_i (own slot - tile start - sub) is the target itself and is skipped.
+template <bool F32, uint32_t K, bool SELF>
+__device__ __forceinline__ void p2p_tile(const typename P2PForm<F32>::T4* __restrict__ ts,
+                                         typename P2PForm<F32>::T xi, typename P2PForm<F32>::T yi,
+                                         typename P2PForm<F32>::T zi,
+                                         typename P2PForm<F32>::T soft2, int self_i,
+                                         typename P2PForm<F32>::T& ax,
+                                         typename P2PForm<F32>::T& ay,
+                                         typename P2PForm<F32>::T& az,
+                                         typename P2PForm<F32>::T& ph) {
+  using F = P2PForm<F32>;
+  using T = typename F::T;
+  // FP32: 8 reads in flight per lane; FP64 (twice the registers per entry,
+  // longer rsqrt): 4
+#pragma unroll(F32 ? 8 : 4)
+  for (uint32_t i = 0; i < 64 / K; i++) {
+    const auto sj = ts[i * K];
+    const T dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
+    T r2 = F::fmat(dx, dx, F::fmat(dy, dy, F::fmat(dz, dz, soft2)));
+    T mj = sj.w;
+    if (SELF) {
+      const bool self = (int)(i * K) == self_i;
+      r2 = self ? T(1) : r2;
+      mj = self ? T(0) : mj;
     }
-    // 2) the other source leaves: their slot ranges are staged in LDS, then
-    // packed back to back into full 64-body tiles. The global loads of the
-    // next tile are issued before the current tile is computed, so their
-    // latency hides behind the arithmetic.
-    for (uint32_t seg = jb; seg < je; seg += P2P_RMAX) {
-      const uint32_t nr = min(P2P_RMAX, je - seg);
-      __syncthreads();
-      for (uint32_t q = lane; q < nr; q += 64) {
-        uint2 g = prange[seg + q];
-        if (g.x == ft) g.y = 0;  // the own leaf was done above
-        rng[q] = g;
-      }
-      __syncthreads();
-      uint32_t r = 0, off = 0;
-      TileLoad cur = gather_tile(rng, nr, r, off, lane, body, pad);
-      while (cur.fill > 0) {
-        tile[lane] = cur.v;
-        __syncthreads();
-        cur = gather_tile(rng, nr, r, off, lane, body, pad);
-        p2p_tile<false>(tile, sub, K, xi, yi, zi, soft2, -1, ax, ay, az, ph);
-        __syncthreads();
-      }
-    }
-    for (uint32_t o = ntp; o < 64; o <<= 1) {
-      ax += __shfl_xor(ax, (int)o, 64);
-      ay += __shfl_xor(ay, (int)o, 64);
-      az += __shfl_xor(az, (int)o, 64);
-      ph += __shfl_xor(ph, (int)o, 64);
-    }
-    if (sub == 0 && tvalid) {
-      if (it.w == P2P_DIRECT) {
-        const double4 fv = far[si];
-        const uint32_t id = order[si];
-        accel[3 * (size_t)id] = ax + fv.x;
-        accel[3 * (size_t)id + 1] = ay + fv.y;
-        accel[3 * (size_t)id + 2] = az + fv.z;
-        if (pot) pot[id] = ph + fv.w;
-      } else {
-        partial[(size_t)it.w + t0 + ti] = make_double4(ax, ay, az, ph);
-      }
-    }
+    // r2 is finite and > 0 here (softening, or distinct bodies, or padding)
+    const T inv = F::rsq(r2);
+    const T minv = mj * inv;
+    const T s3 = minv * inv * inv;
+    ax = F::fmat(s3, dx, ax);
+    ay = F::fmat(s3, dy, ay);
+    az = F::fmat(s3, dz, az);
+    ph += minv;
   }
 }
 
-// ----------------------------------------------------------- FP32 bulk form
-// Same traversal as k_p2p; sources are staged as float4 positions relative
-// to the target leaf's centre (the subtraction in FP64), the pairwise terms
-// run in FP32 with the hardware rsqrt, and each tile's FP32 partial sums are
-// added into FP64 accumulators.
-template <bool SELF>
-__device__ __forceinline__ void p2p_tile32(const float4* __restrict__ tile, uint32_t sub,
-                                           uint32_t K, float xi, float yi, float zi, float soft2,
-                                           int self_j, float& ax, float& ay, float& az,
-                                           float& ph) {
-#pragma unroll 4
-  for (uint32_t j = sub; j < 64; j += K) {
-    const float4 sj = tile[j];
-    const float dx = sj.x - xi, dy = sj.y - yi, dz = sj.z - zi;
-    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
-    float mj = sj.w;
-    if (SELF) {
-      const bool self = (int)j == self_j;
-      r2 = self ? 1.0f : r2;
-      mj = self ? 0.0f : mj;
-    }
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    const float minv = mj * inv;
-    const float s3 = minv * inv * inv;
-    ax = fmaf(s3, dx, ax);
-    ay = fmaf(s3, dy, ay);
-    az = fmaf(s3, dz, az);
-    ph += minv;
+// One chunk's pass over a staged tile: n targets (a power of two), 64 / n
+// lanes per target, n iterations per lane.
+template <bool F32, bool SELF>
+__device__ __forceinline__ void p2p_chunk(const typename P2PForm<F32>::T4* __restrict__ tile,
+                                          uint32_t n, uint32_t sub, typename P2PForm<F32>::T xi,
+                                          typename P2PForm<F32>::T yi,
+                                          typename P2PForm<F32>::T zi,
+                                          typename P2PForm<F32>::T soft2, int self_i,
+                                          typename P2PForm<F32>::T& ax,
+                                          typename P2PForm<F32>::T& ay,
+                                          typename P2PForm<F32>::T& az,
+                                          typename P2PForm<F32>::T& ph) {
+  const auto* ts = tile + sub;
+  switch (n) {
+    case 64: p2p_tile<F32, 1, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    case 32: p2p_tile<F32, 2, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    case 16: p2p_tile<F32, 4, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    case 8: p2p_tile<F32, 8, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    case 4: p2p_tile<F32, 16, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    case 2: p2p_tile<F32, 32, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+    default: p2p_tile<F32, 64, SELF>(ts, xi, yi, zi, soft2, self_i, ax, ay, az, ph); break;
+  }
+}
+
+// A group of r <= 64 targets of one leaf is cut into at most P2P_NQ chunks of
+// power-of-two sizes n_q (64 / n_q lanes per target), all run on every
+// staged source tile: a tile costs sum(n_q) iterations per lane, so a leaf
+// of 37 bodies runs as 32 + 4 + 1 instead of one 64-target chunk with 27
+// idle targets. The plan weighs the padding against a fixed cost per chunk
+// and tile (its sums and loop set-up), in VALU issues.
+constexpr int P2P_NQ = 3;
+
+__device__ __forceinline__ uint32_t pow2ceil(uint32_t x) {
+  return x <= 1u ? 1u : 1u << (32 - __clz(x - 1u));
+}
+__device__ __forceinline__ uint32_t pow2floor(uint32_t x) { return 1u << (31 - __clz(x)); }
+
+__device__ __forceinline__ void chunk_plan(uint32_t r, uint32_t (&n)[P2P_NQ]) {
+  constexpr uint32_t IT = 14, CH = 12;  // per iteration, per chunk
+  n[0] = pow2ceil(r);
+  n[1] = n[2] = 0;
+  uint32_t best = IT * n[0] + CH;
+  const uint32_t a = pow2floor(r), ra = r - a;
+  if (ra == 0) return;
+  const uint32_t b2 = pow2ceil(ra);
+  if (IT * (a + b2) + 2 * CH < best) {
+    best = IT * (a + b2) + 2 * CH;
+    n[0] = a;
+    n[1] = b2;
+  }
+  const uint32_t b = pow2floor(ra), rb = ra - b;
+  if (rb == 0) return;
+  const uint32_t c = pow2ceil(rb);
+  if (IT * (a + b + c) + 3 * CH < best) {
+    n[0] = a;
+    n[1] = b;
+    n[2] = c;
   }
 }
 
@@ -210,110 +191,163 @@ __device__ __forceinline__ void acc2(double& s, double& c, double v) {
   s = t;
 }
 
-__device__ __forceinline__ float4 rel32(double4 v, double cx, double cy, double cz) {
-  return make_float4((float)(v.x - cx), (float)(v.y - cy), (float)(v.z - cz), (float)v.w);
-}
-
-__global__ void __launch_bounds__(64) k_p2p_f32(const uint4* __restrict__ items,
-                                               const uint2* __restrict__ prange,
-                                               const uint32_t* __restrict__ cfirst,
-                                               const uint32_t* __restrict__ ccount,
-                                               const double* __restrict__ ccx,
-                                               const double* __restrict__ ccy,
-                                               const double* __restrict__ ccz,
-                                               const double4* __restrict__ body, double soft2,
-                                               const double4* __restrict__ far,
-                                               const uint32_t* __restrict__ order,
-                                               double* __restrict__ accel, double* __restrict__ pot,
-                                               double4* __restrict__ partial) {
-  __shared__ float4 tile[64];
-  __shared__ uint2 rng[P2P_RMAX];
+// One wave per work item. The targets of the item's leaf are taken in groups
+// of 64, each group cut into chunks (chunk_plan); the item's source stream
+// (the leaf's own bodies first when the item holds them, each target
+// skipping its own slot there; then the neighbour leaves' ranges with the own
+// leaf's entry emptied) is packed back to back into 64-body LDS tiles, and
+// every chunk runs on every tile. The next tile's global loads are issued
+// before the current one is computed. The mixed form (F32) sums each tile in
+// FP32 and compensates the final sums (sub-lane partials, then the far field).
+template <bool F32>
+__global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
+                                           const uint2* __restrict__ prange,
+                                           const uint32_t* __restrict__ cfirst,
+                                           const uint32_t* __restrict__ ccount,
+                                           const double* __restrict__ ccx,
+                                           const double* __restrict__ ccy,
+                                           const double* __restrict__ ccz,
+                                           const double4* __restrict__ body, double soft2,
+                                           const double4* __restrict__ far,
+                                           const uint32_t* __restrict__ order,
+                                           double* __restrict__ accel, double* __restrict__ pot,
+                                           double4* __restrict__ partial) {
+  using F = P2PForm<F32>;
+  using T = typename F::T;
+  __shared__ typename F::T4 tile[64];
+  __shared__ uint2 rng[P2P_RMAX + 1];
   const uint32_t lane = threadIdx.x;
   const uint4 it = items[xcd_block(blockIdx.x, gridDim.x)];
   const uint32_t leaf = it.x & P2P_LEAF_MASK;
   const bool own = (it.x & P2P_OWN) != 0u;
   const uint32_t ft = cfirst[leaf], nt_all = ccount[leaf];
   const uint32_t jb = it.y, je = it.z;
-  const double cx = ccx[leaf], cy = ccy[leaf], cz = ccz[leaf];
+  // the mixed form works relative to the leaf's centre; FP64 in place
+  const double cx = F32 ? ccx[leaf] : 0.0, cy = F32 ? ccy[leaf] : 0.0,
+               cz = F32 ? ccz[leaf] : 0.0;
   const double4 pad = make_double4(cx + 1e18, cy + 1e18, cz + 1e18, 0.0);
-  const float s2f = (float)soft2;
+  const T s2 = (T)soft2;
+  const uint32_t nseg = (je - jb + P2P_RMAX - 1) / P2P_RMAX;
   for (uint32_t t0 = 0; t0 < nt_all; t0 += 64) {
-    const uint32_t nt = min(64u, nt_all - t0);
-    uint32_t ntp = 1;
-    while (ntp < nt) ntp <<= 1;
-    const uint32_t K = 64 / ntp;
-    const uint32_t ti = lane & (ntp - 1);
-    const uint32_t sub = lane / ntp;
-    const bool tvalid = ti < nt;
-    const uint32_t si = ft + t0 + (tvalid ? ti : 0);
-    const float4 me = rel32(body[si], cx, cy, cz);
-    double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;  // FP64 sums of the FP32 tile sums
-    for (uint32_t s0 = 0; own && s0 < nt_all; s0 += 64) {
-      __syncthreads();
-      const bool in = s0 + lane < nt_all;
-      double4 v = body[in ? ft + s0 + lane : ft];
-      if (!in) v = pad;
-      tile[lane] = rel32(v, cx, cy, cz);
-      __syncthreads();
-      const int self_j = (int)(t0 + ti) - (int)s0;
-      float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
-      p2p_tile32<true>(tile, sub, K, me.x, me.y, me.z, s2f, self_j, fx, fy, fz, fp);
-      ax += fx;
-      ay += fy;
-      az += fz;
-      ph += fp;
+    const uint32_t r = min(64u, nt_all - t0);
+    uint32_t n[P2P_NQ];
+    chunk_plan(r, n);
+    // lane state per chunk: target index within the leaf (t0 + base + ti)
+    // and the lane's strided share sub of each tile (64 / n lanes per target)
+    uint32_t tix[P2P_NQ], sub[P2P_NQ];
+    bool tval[P2P_NQ];
+    T mx[P2P_NQ], my[P2P_NQ], mz[P2P_NQ];
+    double ax[P2P_NQ], ay[P2P_NQ], az[P2P_NQ], ph[P2P_NQ];
+    uint32_t base = 0;
+#pragma unroll
+    for (int q = 0; q < P2P_NQ; q++) {
+      const uint32_t nq = n[q] ? n[q] : 1u;
+      const uint32_t ti = lane & (nq - 1);
+      sub[q] = lane / nq;
+      tval[q] = n[q] != 0 && base + ti < r;
+      tix[q] = t0 + base + ti;
+      const auto me = F::stage(body[ft + (tval[q] ? tix[q] : 0)], cx, cy, cz);
+      mx[q] = me.x;
+      my[q] = me.y;
+      mz[q] = me.z;
+      ax[q] = ay[q] = az[q] = ph[q] = 0.0;
+      base += n[q];
     }
-    for (uint32_t seg = jb; seg < je; seg += P2P_RMAX) {
-      const uint32_t nr = min(P2P_RMAX, je - seg);
+    const uint32_t npass = max(nseg, own ? 1u : 0u);
+    for (uint32_t sg = 0; sg < npass; sg++) {
+      const uint32_t seg = jb + sg * P2P_RMAX;
+      const uint32_t nr = seg < je ? min(P2P_RMAX, je - seg) : 0u;
+      const bool pre = own && sg == 0;  // stream starts with the own leaf
       __syncthreads();
+      if (lane == 0) rng[0] = make_uint2(ft, pre ? nt_all : 0u);
       for (uint32_t q = lane; q < nr; q += 64) {
         uint2 g = prange[seg + q];
         if (g.x == ft) g.y = 0;
-        rng[q] = g;
+        rng[1 + q] = g;
       }
       __syncthreads();
-      uint32_t r = 0, off = 0;
-      TileLoad cur = gather_tile(rng, nr, r, off, lane, body, pad);
+      uint32_t rr = 0, off = 0, pos = 0;
+      TileLoad cur = gather_tile(rng, nr + 1, rr, off, lane, body, pad);
       while (cur.fill > 0) {
-        tile[lane] = rel32(cur.v, cx, cy, cz);
+        tile[lane] = F::stage(cur.v, cx, cy, cz);
+        const uint32_t fill = cur.fill;
         __syncthreads();
-        cur = gather_tile(rng, nr, r, off, lane, body, pad);
-        float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
-        p2p_tile32<false>(tile, sub, K, me.x, me.y, me.z, s2f, -1, fx, fy, fz, fp);
-        ax += fx;
-        ay += fy;
-        az += fz;
-        ph += fp;
+        cur = gather_tile(rng, nr + 1, rr, off, lane, body, pad);
+        const bool self_tile = pre && pos < nt_all;
+#pragma unroll
+        for (int q = 0; q < P2P_NQ; q++) {
+          if (n[q] == 0) break;
+          const int self_i = (int)tix[q] - (int)pos - (int)sub[q];
+          if constexpr (F32) {
+            float fx = 0.f, fy = 0.f, fz = 0.f, fp = 0.f;
+            if (self_tile)
+              p2p_chunk<true, true>(tile, n[q], sub[q], mx[q], my[q], mz[q], s2, self_i, fx, fy,
+                                    fz, fp);
+            else
+              p2p_chunk<true, false>(tile, n[q], sub[q], mx[q], my[q], mz[q], s2, 0, fx, fy, fz,
+                                     fp);
+            ax[q] += fx;
+            ay[q] += fy;
+            az[q] += fz;
+            ph[q] += fp;
+          } else {
+            if (self_tile)
+              p2p_chunk<false, true>(tile, n[q], sub[q], mx[q], my[q], mz[q], s2, self_i, ax[q],
+                                     ay[q], az[q], ph[q]);
+            else
+              p2p_chunk<false, false>(tile, n[q], sub[q], mx[q], my[q], mz[q], s2, 0, ax[q],
+                                      ay[q], az[q], ph[q]);
+          }
+        }
+        pos += fill;
         __syncthreads();
       }
     }
-    // the final per-body sum (sub-lane partials, then the far field) is
-    // compensated
-    double kx = 0.0, ky = 0.0, kz = 0.0, kp = 0.0;
-    for (uint32_t o = ntp; o < 64; o <<= 1) {
-      acc2(ax, kx, __shfl_xor(ax, (int)o, 64));
-      acc2(ay, ky, __shfl_xor(ay, (int)o, 64));
-      acc2(az, kz, __shfl_xor(az, (int)o, 64));
-      acc2(ph, kp, __shfl_xor(ph, (int)o, 64));
-      kx += __shfl_xor(kx, (int)o, 64);
-      ky += __shfl_xor(ky, (int)o, 64);
-      kz += __shfl_xor(kz, (int)o, 64);
-      kp += __shfl_xor(kp, (int)o, 64);
-    }
-    if (sub == 0 && tvalid) {
-      if (it.w == P2P_DIRECT) {  // the final near + far sum, compensated
-        const double4 fv = far[si];
-        const uint32_t id = order[si];
-        acc2(ax, kx, fv.x);
-        acc2(ay, ky, fv.y);
-        acc2(az, kz, fv.z);
-        acc2(ph, kp, fv.w);
-        accel[3 * (size_t)id] = ax + kx;
-        accel[3 * (size_t)id + 1] = ay + ky;
-        accel[3 * (size_t)id + 2] = az + kz;
-        if (pot) pot[id] = ph + kp;
-      } else {
-        partial[(size_t)it.w + t0 + ti] = make_double4(ax + kx, ay + ky, az + kz, ph + kp);
+#pragma unroll
+    for (int q = 0; q < P2P_NQ; q++) {
+      if (n[q] == 0) break;
+      double kx = 0.0, ky = 0.0, kz = 0.0, kp = 0.0;
+      for (uint32_t o = n[q]; o < 64; o <<= 1) {
+        if constexpr (F32) {
+          acc2(ax[q], kx, __shfl_xor(ax[q], (int)o, 64));
+          acc2(ay[q], ky, __shfl_xor(ay[q], (int)o, 64));
+          acc2(az[q], kz, __shfl_xor(az[q], (int)o, 64));
+          acc2(ph[q], kp, __shfl_xor(ph[q], (int)o, 64));
+          kx += __shfl_xor(kx, (int)o, 64);
+          ky += __shfl_xor(ky, (int)o, 64);
+          kz += __shfl_xor(kz, (int)o, 64);
+          kp += __shfl_xor(kp, (int)o, 64);
+        } else {
+          ax[q] += __shfl_xor(ax[q], (int)o, 64);
+          ay[q] += __shfl_xor(ay[q], (int)o, 64);
+          az[q] += __shfl_xor(az[q], (int)o, 64);
+          ph[q] += __shfl_xor(ph[q], (int)o, 64);
+        }
+      }
+      if (sub[q] == 0 && tval[q]) {
+        const uint32_t si = ft + tix[q];
+        if (it.w == P2P_DIRECT) {  // the final near + far sum
+          const double4 fv = far[si];
+          const uint32_t id = order[si];
+          if constexpr (F32) {
+            acc2(ax[q], kx, fv.x);
+            acc2(ay[q], ky, fv.y);
+            acc2(az[q], kz, fv.z);
+            acc2(ph[q], kp, fv.w);
+          } else {
+            ax[q] += fv.x;
+            ay[q] += fv.y;
+            az[q] += fv.z;
+            ph[q] += fv.w;
+          }
+          accel[3 * (size_t)id] = ax[q] + kx;
+          accel[3 * (size_t)id + 1] = ay[q] + ky;
+          accel[3 * (size_t)id + 2] = az[q] + kz;
+          if (pot) pot[id] = ph[q] + kp;
+        } else {
+          partial[(size_t)it.w + tix[q]] =
+              make_double4(ax[q] + kx, ay[q] + ky, az[q] + kz, ph[q] + kp);
+        }
       }
     }
   }
@@ -547,7 +581,7 @@ void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const ui
                     double soft2, const double4* far, const uint32_t* order, double* accel,
                     double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
-  hipLaunchKernelGGL(k_p2p_f32, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
+  hipLaunchKernelGGL(k_p2p<true>, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
@@ -566,8 +600,8 @@ void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2*
                 double soft2, const double4* far, const uint32_t* order, double* accel,
                 double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
-  hipLaunchKernelGGL(k_p2p, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first, c.count,
-                     body, soft2, far, order, accel, pot, pl.partial);
+  hipLaunchKernelGGL(k_p2p<false>, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
+                     c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
 __global__ void __launch_bounds__(256) k_p2p_count(const uint2* __restrict__ pairs, uint32_t np,
