@@ -612,8 +612,8 @@ def main():
                     help="leaf capacities the autotuner tries (with --n-leaf 0); default "
                          "64,96,128,192 at 16M bodies, 16..256 below")
     ap.add_argument("--p-grid", default=None,
-                    help="expansion orders the autotuner tries (with --n-leaf 0); default 6,8 at "
-                         "16M bodies, 6,8,10 below")
+                    help="expansion orders the autotuner tries (with --n-leaf 0); default 6,7,8 "
+                         "at 16M bodies, 6..10 below")
     ap.add_argument("--chunk-grid", default="16,32,64",
                     help="M2L chunk lengths the autotuner tries (with --n-leaf 0)")
     ap.add_argument("--item-grid", default="65536,262144,1048576",
@@ -658,7 +658,7 @@ def main():
     if args.leaf_grid is None:
         args.leaf_grid = "64,96,128,192" if big else "16,24,32,48,64,96,128,192,256"
     if args.p_grid is None:
-        args.p_grid = "6,8" if big else "6,8,10"
+        args.p_grid = "6,7,8" if big else "6,7,8,9,10"
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
