@@ -108,7 +108,7 @@ struct fmmh_ctx {
   Buf scan_tmp, sort_tmp;
   Buf moments, locals, pows, fvals, fmin, fact, binom;
   Buf front_a, front_b, m2l, p2p, ctr;
-  Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, pcount;
+  Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr;
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
   Buf walk_code, walk_blk;  // the walk's per-pair decisions, per-block counts
@@ -133,7 +133,7 @@ struct fmmh_ctx {
                   &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
                   &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
-                  &pcount, &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &walk_code, &walk_blk, &pl_n, &pl_items,
+                  &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &walk_code, &walk_blk, &pl_n, &pl_items,
                   &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &loc32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &direct_part, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
                   &d_sysoff};
@@ -491,7 +491,6 @@ struct fmmh_ctx {
   // Directed lists grouped by target, P2P work count, M2L chunks.
   void build_lists() {
     const int p = prm.p;
-    const int kbits = 2 * sbits;
     ensure(mkeys, (size_t)2 * n_m2l * 8);
     ensure(mkeys_alt, (size_t)2 * n_m2l * 8);
     ensure(tptr, ((size_t)nc + 1) * 4);
@@ -499,24 +498,21 @@ struct fmmh_ctx {
     // The walk writes its pairs in one fixed order, so a stable sort on the
     // target bits alone groups the directed entries deterministically (each
     // target's sources in walk order).
-    launch_directed_keys_m2l((uint2*)m2l.p, n_m2l, sbits, (uint64_t*)mkeys.p, s);
-    int fl = radix_sort_u64((uint64_t*)mkeys.p, nullptr, (uint64_t*)mkeys_alt.p, nullptr,
-                            (size_t)2 * n_m2l, sbits, kbits, (uint32_t*)sort_tmp.p, s);
+    int fl = radix_sort_directed((const uint2*)m2l.p, n_m2l, sbits, false, (uint64_t*)mkeys.p,
+                                 (uint64_t*)mkeys_alt.p, (uint32_t*)sort_tmp.p, s);
     msorted = (const uint64_t*)(fl ? mkeys_alt.p : mkeys.p);
     launch_csr_ptr(msorted, 2 * n_m2l, sbits, nc, (uint32_t*)tptr.p, s);
     ensure(pkeys, (size_t)2 * n_p2p * 8);
     ensure(pkeys_alt, (size_t)2 * n_p2p * 8);
     ensure(pptr, ((size_t)nc + 1) * 4);
-    ensure(pcount, 64);
     ensure(sort_tmp, radix_tmp_words((size_t)2 * n_p2p) * 4 + 64);
-    HCHECK(hipMemsetAsync(pcount.p, 0, 4, s));
-    launch_directed_keys_p2p((uint2*)p2p.p, n_p2p, sbits, (uint64_t*)pkeys.p, (uint32_t*)pcount.p, s);
-    fl = radix_sort_u64((uint64_t*)pkeys.p, nullptr, (uint64_t*)pkeys_alt.p, nullptr,
-                        (size_t)2 * n_p2p, sbits, kbits, (uint32_t*)sort_tmp.p, s);
+    // a self pair's second entry is a sentinel that sorts past every real
+    // entry: the row pointers stop before them, and ptr[nc] counts the rest
+    fl = radix_sort_directed((const uint2*)p2p.p, n_p2p, sbits, true, (uint64_t*)pkeys.p,
+                             (uint64_t*)pkeys_alt.p, (uint32_t*)sort_tmp.p, s);
     psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
-    uint32_t nself = read1((const uint32_t*)pcount.p);
-    pnnz = 2 * n_p2p - nself;
-    launch_csr_ptr(psorted, pnnz, sbits, nc, (uint32_t*)pptr.p, s);
+    launch_csr_ptr(psorted, 2 * n_p2p, sbits, nc, (uint32_t*)pptr.p, s);
+    pnnz = read1((const uint32_t*)pptr.p + nc);
     ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
     launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
                      (unsigned long long*)p2p_work.p, s);

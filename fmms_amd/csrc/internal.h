@@ -18,6 +18,11 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
 size_t radix_tmp_words(size_t n);
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* k_alt, uint32_t* v_alt, size_t n,
                    int bit_lo, int bit_hi, uint32_t* tmp, hipStream_t s);
+// Directed entries of undirected pairs (x, y): 2i -> (y << sbits | x), 2i+1
+// -> (x << sbits | y), or with `sentinel` all ones in [0, 2 sbits) for a
+// self pair's second entry; sorted stably by the target bits.
+int radix_sort_directed(const uint2* pairs, uint32_t np, int sbits, bool sentinel, uint64_t* keys,
+                        uint64_t* k_alt, uint32_t* tmp, hipStream_t s);
 
 // Cells in DFS preorder (tree.rs:49-58 as structure of arrays).
 struct DevCells {
@@ -145,11 +150,6 @@ void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f
                       uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, const uint16_t* code,
                       const uint32_t* blk, uint32_t m2l0, uint32_t p2p0, WalkResult* res,
                       const void* rec, hipStream_t s);
-void launch_directed_keys_m2l(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
-                              hipStream_t s);
-// returns via ctr->next the number of directed P2P entries written
-void launch_directed_keys_p2p(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
-                              uint32_t* count, hipStream_t s);
 void launch_csr_ptr(const uint64_t* keys, uint32_t nnz, int sbits, uint32_t ntarget,
                     uint32_t* ptr, hipStream_t s);
 

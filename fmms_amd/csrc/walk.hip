@@ -11,6 +11,8 @@
 // come out in one fixed order). The MAC (mac.rs:23-95) is evaluated in the
 // reference's operation order, so decisions agree bit for bit.
 
+#include <algorithm>
+
 #include "common.h"
 #include "internal.h"
 
@@ -318,48 +320,10 @@ void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f
                      m2l, cap_m2l, p2p, cap_p2p, code, blk, m2l0, p2p0, res, (const WalkRec*)rec);
 }
 
-// Directed M2L entries (target << sbits | source), both directions per pair.
-__global__ void __launch_bounds__(256) k_keys_m2l(const uint2* __restrict__ pairs, uint32_t np,
-                                                 int sbits, uint64_t* __restrict__ keys) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= np) return;
-  uint2 pr = pairs[i];
-  keys[2 * (size_t)i] = ((uint64_t)pr.y << sbits) | pr.x;
-  keys[2 * (size_t)i + 1] = ((uint64_t)pr.x << sbits) | pr.y;
-}
-
-void launch_directed_keys_m2l(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
-                              hipStream_t s) {
-  if (npairs == 0) return;
-  hipLaunchKernelGGL(k_keys_m2l, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, npairs, sbits,
-                     keys);
-}
-
-// Directed P2P entries; a self pair (a,a) yields one entry and a sentinel
-// (all ones in the key range) that sorts past every real entry.
-__global__ void __launch_bounds__(256) k_keys_p2p(const uint2* __restrict__ pairs, uint32_t np,
-                                                 int sbits, uint64_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ count) {
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= np) return;
-  uint2 pr = pairs[i];
-  keys[2 * (size_t)i] = ((uint64_t)pr.y << sbits) | pr.x;
-  if (pr.x == pr.y) {
-    keys[2 * (size_t)i + 1] = (1ull << (2 * sbits)) - 1ull;
-    atomicAdd(count, 1u);
-  } else {
-    keys[2 * (size_t)i + 1] = ((uint64_t)pr.x << sbits) | pr.y;
-  }
-}
-
-void launch_directed_keys_p2p(const uint2* pairs, uint32_t npairs, int sbits, uint64_t* keys,
-                              uint32_t* count, hipStream_t s) {
-  if (npairs == 0) return;
-  hipLaunchKernelGGL(k_keys_p2p, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, npairs, sbits,
-                     keys, count);
-}
-
-// Row pointers of the target-sorted directed entries over all cells.
+// Row pointers of the target-sorted directed entries over all cells:
+// ptr[t] = first entry of target t, ptr[ntarget] = entries with a target
+// below ntarget. Entries past those (sentinel targets >= ntarget, sorted
+// last) are not counted.
 __global__ void __launch_bounds__(256) k_csr_ptr(const uint64_t* __restrict__ keys, uint32_t nnz,
                                                 int sbits, uint32_t ntarget,
                                                 uint32_t* __restrict__ ptr) {
@@ -369,11 +333,11 @@ __global__ void __launch_bounds__(256) k_csr_ptr(const uint64_t* __restrict__ ke
     return;
   }
   if (i >= nnz) return;
-  uint32_t t = (uint32_t)(keys[i] >> sbits);
-  int64_t tp = i == 0 ? -1 : (int64_t)(keys[i - 1] >> sbits);
-  for (int64_t x = tp + 1; x <= (int64_t)t; x++) ptr[x] = i;
+  const int64_t t = std::min<int64_t>((int64_t)(keys[i] >> sbits), (int64_t)ntarget);
+  const int64_t tp = i == 0 ? -1 : std::min<int64_t>((int64_t)(keys[i - 1] >> sbits), ntarget);
+  for (int64_t x = tp + 1; x <= t; x++) ptr[x] = i;
   if (i == nnz - 1)
-    for (uint32_t x = t + 1; x <= ntarget; x++) ptr[x] = nnz;
+    for (int64_t x = t + 1; x <= (int64_t)ntarget; x++) ptr[x] = nnz;
 }
 
 void launch_csr_ptr(const uint64_t* keys, uint32_t nnz, int sbits, uint32_t ntarget, uint32_t* ptr,
