@@ -85,6 +85,7 @@ struct fmmh_ctx {
   uint64_t* keys_sorted = nullptr;
   uint32_t* order = nullptr;
   uint32_t n_m2l = 0, n_p2p = 0, walk_rounds = 0;
+  uint32_t walk_batch = 8;  // walk rounds launched per host check
   uint64_t p2p_interactions = 0;
   // timing
   hipEvent_t ev[ST_COUNT + 1] = {};
@@ -847,38 +848,46 @@ struct fmmh_ctx {
     mp.soft2 = prm.softening * prm.softening;
     mp.soft_mac = prm.soft_mac;
     // the walk starts from the self pair of every (system) root: level vlev
-    uint32_t nf = lvl_off[vlev + 1] - lvl_off[vlev];
-    launch_front_init((const uint32_t*)lvl_cells.p + lvl_off[vlev], nf, front_a.p, s);
-    WalkResult* res = (WalkResult*)ctr.p;
-    HCHECK(hipMemsetAsync(res, 0, sizeof(WalkResult), s));
-    n_m2l = n_p2p = 0;
-    walk_rounds = 0;
-    Buf* fin = &front_a;
-    Buf* fout = &front_b;
+    const uint32_t nf0 = lvl_off[vlev + 1] - lvl_off[vlev];
+    WalkState* st = (WalkState*)ctr.p;
+    HCHECK(hipMemsetAsync(st, 0, sizeof(WalkState), s));
+    launch_front_init((const uint32_t*)lvl_cells.p + lvl_off[vlev], nf0, front_a.p, st, s);
+    ensure(walk_blk, (size_t)3 * WALK_G * 4);
+    Buf* fr[2] = {&front_a, &front_b};
     auto cap = [](const Buf& b) { return (uint32_t)std::min<size_t>(b.cap / 8, 0xFFFFFFFFu); };
-    while (nf > 0) {
-      ensure(walk_code, (size_t)nf * 2);
-      ensure(walk_blk, (size_t)walk_blocks(nf) * 12);
-      launch_walk_decide((const uint2*)fin->p, nf, (uint16_t*)walk_code.p, (uint32_t*)walk_blk.p,
-                         walkrec.p, (const double*)c_mass.p, mp, res, s);
-      WalkResult r;
-      for (;;) {
-        launch_walk_emit((const uint2*)fin->p, nf, (uint2*)fout->p, cap(*fout), (uint2*)m2l.p,
-                         cap(m2l), (uint2*)p2p.p, cap(p2p), (const uint16_t*)walk_code.p,
-                         (const uint32_t*)walk_blk.p, n_m2l, n_p2p, res, walkrec.p, s);
-        r = read1((const WalkResult*)res);
-        if (r.status == 0) break;
-        if (r.status & 1u) ensure(*fout, (size_t)r.next * 8 * 2);
-        if (r.status & 2u) ensure(m2l, (size_t)r.n_m2l * 8 * 2, true);
-        if (r.status & 4u) ensure(p2p, (size_t)r.n_p2p * 8 * 2, true);
-        HCHECK(hipMemsetAsync(&res->status, 0, sizeof(uint32_t), s));
+    // Rounds are launched in batches (as many as the last walk took, plus
+    // one) and the host reads the walk state once per batch: the walk is
+    // done when the frontier of the next round is empty; a halted round
+    // (a buffer too small) resumes after the buffers have grown.
+    uint32_t r = 0;
+    for (;;) {
+      ensure(walk_code, (size_t)std::max(cap(front_a), cap(front_b)) * 2);
+      const uint32_t batch = std::max<uint32_t>(4u, walk_batch);
+      for (uint32_t k = 0; k < batch; k++, r++)
+        launch_walk_round(fr[r & 1u]->p, fr[(r + 1u) & 1u]->p, r, st, cap(*fr[(r + 1u) & 1u]),
+                          (uint2*)m2l.p, cap(m2l), (uint2*)p2p.p, cap(p2p),
+                          (uint16_t*)walk_code.p, (uint32_t*)walk_blk.p, walkrec.p,
+                          (const double*)c_mass.p, mp, s);
+      const WalkState h = read1((const WalkState*)st);
+      r = h.round;
+      if (h.status != 0u) {
+        // round r needs more room: its input frontier is kept, the rest grows
+        if (h.status & 1u) {
+          ensure(*fr[(r + 1u) & 1u], (size_t)h.need_f * 8 * 2);
+          ensure(*fr[r & 1u], fr[(r + 1u) & 1u]->cap, true);
+        }
+        if (h.status & 2u) ensure(m2l, (size_t)h.need_m2l * 8 * 2, true);
+        if (h.status & 4u) ensure(p2p, (size_t)h.need_p2p * 8 * 2, true);
+        HCHECK(hipMemsetAsync(&st->status, 0, sizeof(uint32_t), s));
+        continue;
       }
-      n_m2l = r.n_m2l;
-      n_p2p = r.n_p2p;
-      nf = r.next;
-      std::swap(fin, fout);
-      walk_rounds++;
-      if (fin->cap > fout->cap) ensure(*fout, fin->cap);
+      if (h.fsize[r & 1u] == 0u) {
+        n_m2l = h.n_m2l;
+        n_p2p = h.n_p2p;
+        walk_rounds = h.rounds;
+        walk_batch = h.rounds + 1u;
+        break;
+      }
     }
   }
 };

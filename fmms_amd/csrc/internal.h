@@ -130,26 +130,33 @@ struct MacParams {
   double soft_mac;  // softening guard tau (0 = off)
 };
 size_t walk_rec_bytes();
-void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s);
+// Device state of the breadth-first walk (walk.hip). Round r reads the
+// frontier size fsize[r & 1] and writes fsize[(r + 1) & 1]; m2l_base /
+// p2p_base are the list lengths at the start of the round being emitted.
+// status bits 1 / 2 / 4 (frontier / M2L / P2P buffer too small for round
+// `round`, need_* the sizes it needs) halt the walk at that round: the round
+// and every later one leave all outputs and the state untouched, so the walk
+// resumes at `round` once the buffers have grown.
+struct WalkState {
+  uint32_t fsize[2];
+  uint32_t n_m2l, n_p2p;
+  uint32_t m2l_base, p2p_base;
+  uint32_t round;   // next round to run
+  uint32_t rounds;  // rounds done with a non-empty frontier
+  uint32_t status;
+  uint32_t need_f, need_m2l, need_p2p;
+};
+constexpr uint32_t WALK_G = 2048;  // blocks per walk launch (blk holds 3 x WALK_G words)
+void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, WalkState* st,
+                       hipStream_t s);
 // Walk records, and {cx, cy, cz, mass} of every cell as one 32-byte record.
 void launch_pack_walk(uint32_t nc, DevCells c, void* rec, double4* cpm, hipStream_t s);
-// One round of the walk (walk.hip) in frontier order: decide + per-block
-// counts + their scan (res->next = the next frontier's size, res->n_m2l /
-// n_p2p advanced by the round's pairs), then the outputs (status bits 1 / 2
-// / 4 = frontier / M2L / P2P capacity exceeded: nothing of that kind was
-// written, run the emit again with more room). blk holds 3 x walk_blocks(nf)
-// words; code nf.
-struct WalkResult {
-  uint32_t next, n_m2l, n_p2p, status;
-};
-uint32_t walk_blocks(uint32_t nf);
-void launch_walk_decide(const uint2* fin, uint32_t nf, uint16_t* code, uint32_t* blk,
-                        const void* rec, const double* mass, MacParams mac, WalkResult* res,
-                        hipStream_t s);
-void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
-                      uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, const uint16_t* code,
-                      const uint32_t* blk, uint32_t m2l0, uint32_t p2p0, WalkResult* res,
-                      const void* rec, hipStream_t s);
+// Round r of the walk: decide, scan + state update, emit (fin / fout: the
+// frontier buffers of round r; code holds at least the frontier capacity).
+void launch_walk_round(const void* fin, void* fout, uint32_t r, WalkState* st, uint32_t cap_f,
+                       uint2* m2l, uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, uint16_t* code,
+                       uint32_t* blk, const void* rec, const double* mass, MacParams mac,
+                       hipStream_t s);
 void launch_csr_ptr(const uint64_t* keys, uint32_t nnz, int sbits, uint32_t ntarget,
                     uint32_t* ptr, hipStream_t s);
 

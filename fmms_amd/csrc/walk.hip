@@ -8,7 +8,9 @@
 // walks it breadth-first, one thread per frontier pair per round: children
 // are written to the next frontier and M2L / P2P pairs to their lists at
 // offsets from scans of the per-block counts, in frontier order (so the lists
-// come out in one fixed order). The MAC (mac.rs:23-95) is evaluated in the
+// come out in one fixed order). Each round runs on fixed grids that read the
+// frontier size from device memory, so the host checks the walk's progress
+// only once per batch of rounds. The MAC (mac.rs:23-95) is evaluated in the
 // reference's operation order, so decisions agree bit for bit.
 
 #include <algorithm>
@@ -45,16 +47,19 @@ __global__ void __launch_bounds__(256) k_pack_walk(uint32_t nc, DevCells c,
 size_t walk_rec_bytes() { return sizeof(WalkRec); }
 
 // Initial walk frontier: the self pairs (c, c) of the given cells (the root,
-// or every system root of an ensemble).
+// or every system root of an ensemble); its size goes to the walk state.
 __global__ void __launch_bounds__(256) k_front_init(const uint32_t* __restrict__ cells,
-                                                   uint32_t cnt, uint2* __restrict__ front) {
+                                                   uint32_t cnt, uint2* __restrict__ front,
+                                                   WalkState* __restrict__ st) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < cnt) front[i] = make_uint2(cells[i], cells[i]);
+  if (i == 0) st->fsize[0] = cnt;
 }
 
-void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, hipStream_t s) {
+void launch_front_init(const uint32_t* cells, uint32_t cnt, void* front, WalkState* st,
+                       hipStream_t s) {
   hipLaunchKernelGGL(k_front_init, dim3((cnt + 255) / 256), dim3(256), 0, s, cells, cnt,
-                     (uint2*)front);
+                     (uint2*)front, st);
 }
 
 void launch_pack_walk(uint32_t nc, DevCells c, void* rec, double4* cpm, hipStream_t s) {
@@ -143,6 +148,24 @@ __device__ __forceinline__ int decide(uint32_t a, uint32_t b, const WalkRec& ra,
 
 constexpr int CW_T = 256;
 
+// Sum of three counters over the block (CW_T threads); every thread gets it.
+__device__ __forceinline__ void block_sum3(uint32_t v[3], uint32_t (*sh)[3]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < 3; k++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  }
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) sh[w][k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < 3; k++) {
+    uint32_t t = 0;
+    for (int q = 0; q < CW_T / 64; q++) t += sh[q][k];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
 // Exclusive block scan of three counters (CW_T threads); totals in tot[3].
 __device__ __forceinline__ void block_scan3(const uint32_t v[3], uint32_t x[3], uint32_t tot[3],
                                             uint32_t (*sh)[3]) {
@@ -164,160 +187,179 @@ __device__ __forceinline__ void block_scan3(const uint32_t v[3], uint32_t x[3], 
   __syncthreads();
 }
 
-// One round of the breadth-first walk in three launches, with outputs in
-// frontier order: (1) every frontier pair is decided, its decision stored
-// and its block's output counts written; (2) one block scans the per-block
-// counts into offsets and round totals; (3) every pair writes its outputs
-// (child pairs to the next frontier, M2L and P2P pairs appended to the
-// lists) at its block's offset plus its rank in the block. The lists thus
-// come out in one fixed order, and a system's pairs keep their relative
-// order whether it is walked alone or in an ensemble.
-__global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ fin, uint32_t nf,
+// The round's frontier slice of this block: WALK_G blocks, each a
+// contiguous run of ceil(nf / WALK_G) pairs, in frontier order.
+__device__ __forceinline__ void walk_slice(uint32_t nf, uint32_t* b0, uint32_t* b1) {
+  const uint32_t chunk = (nf + WALK_G - 1) / WALK_G;
+  *b0 = min(blockIdx.x * chunk, nf);
+  *b1 = min(*b0 + chunk, nf);
+}
+
+// One round of the breadth-first walk in three launches of fixed grids, with
+// outputs in frontier order and no host round trip: (1) every pair of the
+// round's frontier (its size read from the walk state) is decided, its
+// decision stored and its block's output counts written; (2) one block scans
+// the per-block counts and, when the round's outputs fit, advances the walk
+// state (next frontier size, list lengths); otherwise it records the sizes
+// needed and halts the walk at this round (every later kernel returns at
+// once); (3) every pair writes its outputs (child pairs to the next
+// frontier, M2L and P2P pairs appended to the lists) at its block's offset
+// plus its rank in the block. The lists thus come out in one fixed order,
+// and a system's pairs keep their relative order whether it is walked alone
+// or in an ensemble.
+__global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ fin, uint32_t r,
+                                                    const WalkState* __restrict__ st,
                                                     uint16_t* __restrict__ code,
                                                     uint32_t* __restrict__ blk,
                                                     const WalkRec* __restrict__ rec,
                                                     const double* __restrict__ mass, MacParams mp) {
   __shared__ uint32_t sh[CW_T / 64][3];
-  const uint32_t t = blockIdx.x * CW_T + threadIdx.x;
+  if (st->status != 0u) return;
+  uint32_t b0, b1;
+  walk_slice(st->fsize[r & 1u], &b0, &b1);
   uint32_t v[3] = {0u, 0u, 0u};  // child pairs, M2L, P2P
-  if (t < nf) {
+  for (uint32_t t = b0 + threadIdx.x; t < b1; t += CW_T) {
     const uint2 pr = fin[t];
     const WalkRec ra = rec[pr.x], rb = rec[pr.y];
     uint32_t nch;
     const int act = decide(pr.x, pr.y, ra, rb, mass, mp, &nch);
     code[t] = (uint16_t)(act | (nch << 3));
-    v[0] = nch;
-    v[1] = act == A_M2L ? 1u : 0u;
-    v[2] = act == A_P2P ? 1u : 0u;
+    v[0] += nch;
+    v[1] += act == A_M2L ? 1u : 0u;
+    v[2] += act == A_P2P ? 1u : 0u;
   }
-  uint32_t x[3], tot[3];
-  block_scan3(v, x, tot, sh);
-  if (threadIdx.x < 3) blk[(size_t)gridDim.x * threadIdx.x + blockIdx.x] = tot[threadIdx.x];
+  block_sum3(v, sh);
+  if (threadIdx.x < 3) blk[WALK_G * threadIdx.x + blockIdx.x] = v[threadIdx.x];
 }
 
-// One block: exclusive scans of the three per-block count columns (nb each,
-// in place), 4096 entries of each column per pass (four per thread); res
-// gets the round totals added to the running list lengths.
-__global__ void __launch_bounds__(1024) k_walk_scan(uint32_t* __restrict__ blk, uint32_t nb,
-                                                  WalkResult* __restrict__ res) {
+// One block of 1024 threads: exclusive scans of the three count columns
+// (WALK_G entries each, in place), then the walk state update.
+__global__ void __launch_bounds__(1024) k_walk_scan(uint32_t* __restrict__ blk, uint32_t r,
+                                                  WalkState* __restrict__ st, uint32_t cap_f,
+                                                  uint32_t cap_m2l, uint32_t cap_p2p) {
+  constexpr int PER = WALK_G / 1024;
   __shared__ uint32_t sh[16][3];
+  if (st->status != 0u) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t carry[3] = {0u, 0u, 0u};
-  for (uint32_t b0 = 0; b0 < nb; b0 += 4096) {
-    const uint32_t j0 = b0 + 4 * threadIdx.x;
-    uint32_t v[3][4], sum[3];
-    for (int k = 0; k < 3; k++) {
-      const uint32_t* col = blk + (size_t)nb * k;
-      sum[k] = 0;
-      for (int q = 0; q < 4; q++) {
-        v[k][q] = j0 + q < nb ? col[j0 + q] : 0u;
-        sum[k] += v[k][q];
-      }
+  uint32_t v[3][PER], sum[3], x[3], wt[3], tot[3];
+  for (int k = 0; k < 3; k++) {
+    sum[k] = 0;
+    for (int q = 0; q < PER; q++) {
+      v[k][q] = blk[WALK_G * k + PER * threadIdx.x + q];
+      sum[k] += v[k][q];
     }
-    uint32_t x[3], wt[3];
-    for (int k = 0; k < 3; k++) x[k] = wave_excl_scan(sum[k], &wt[k]);
-    if (lane == 0)
-      for (int k = 0; k < 3; k++) sh[w][k] = wt[k];
-    __syncthreads();
-    for (int k = 0; k < 3; k++) {
-      uint32_t off = 0, t = 0;
-      for (int q = 0; q < 16; q++) {
-        if (q < w) off += sh[q][k];
-        t += sh[q][k];
-      }
-      uint32_t* col = blk + (size_t)nb * k;
-      uint32_t run = carry[k] + off + x[k];
-      for (int q = 0; q < 4; q++) {
-        if (j0 + q < nb) col[j0 + q] = run;
-        run += v[k][q];
-      }
-      carry[k] += t;
+    x[k] = wave_excl_scan(sum[k], &wt[k]);
+  }
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) sh[w][k] = wt[k];
+  __syncthreads();
+  for (int k = 0; k < 3; k++) {
+    uint32_t off = 0, t = 0;
+    for (int q = 0; q < 16; q++) {
+      if (q < w) off += sh[q][k];
+      t += sh[q][k];
     }
-    __syncthreads();
+    tot[k] = t;
+    uint32_t run = off + x[k];
+    for (int q = 0; q < PER; q++) {
+      blk[WALK_G * k + PER * threadIdx.x + q] = run;
+      run += v[k][q];
+    }
   }
-  if (threadIdx.x == 0) {
-    res->next = carry[0];
-    res->n_m2l += carry[1];
-    res->n_p2p += carry[2];
+  if (threadIdx.x != 0) return;
+  const uint64_t need_m2l = (uint64_t)st->n_m2l + tot[1];
+  const uint64_t need_p2p = (uint64_t)st->n_p2p + tot[2];
+  uint32_t bad = 0u;
+  if (tot[0] > cap_f) bad |= 1u;
+  if (need_m2l > cap_m2l) bad |= 2u;
+  if (need_p2p > cap_p2p) bad |= 4u;
+  if (bad) {
+    st->need_f = tot[0];
+    st->need_m2l = (uint32_t)min(need_m2l, (uint64_t)0xFFFFFFFFu);
+    st->need_p2p = (uint32_t)min(need_p2p, (uint64_t)0xFFFFFFFFu);
+    st->status = bad;
+    return;
   }
+  if (st->fsize[r & 1u] > 0u) st->rounds++;
+  st->fsize[(r + 1u) & 1u] = tot[0];
+  st->m2l_base = st->n_m2l;
+  st->p2p_base = st->n_p2p;
+  st->n_m2l = (uint32_t)need_m2l;
+  st->n_p2p = (uint32_t)need_p2p;
+  st->round = r + 1u;
 }
 
-__global__ void __launch_bounds__(CW_T) k_walk_emit(const uint2* __restrict__ fin, uint32_t nf,
-                                                  uint2* __restrict__ fout, uint32_t cap_f,
-                                                  uint2* __restrict__ m2l, uint32_t cap_m2l,
-                                                  uint2* __restrict__ p2p, uint32_t cap_p2p,
+constexpr uint32_t WALK_STAGE = CW_T * 8;  // child pairs staged per batch
+
+__global__ void __launch_bounds__(CW_T) k_walk_emit(const uint2* __restrict__ fin, uint32_t r,
+                                                  const WalkState* __restrict__ st,
+                                                  uint2* __restrict__ fout, uint2* __restrict__ m2l,
+                                                  uint2* __restrict__ p2p,
                                                   const uint16_t* __restrict__ code,
                                                   const uint32_t* __restrict__ blk,
-                                                  uint32_t m2l0, uint32_t p2p0,
-                                                  WalkResult* __restrict__ res,
                                                   const WalkRec* __restrict__ rec) {
   __shared__ uint32_t sh[CW_T / 64][3];
-  const uint32_t t = blockIdx.x * CW_T + threadIdx.x;
-  const bool valid = t < nf;
-  const uint32_t cd = valid ? (uint32_t)code[t] : 0u;
-  const int act = valid ? (int)(cd & 7u) : A_NONE;
-  const uint32_t v[3] = {cd >> 3, act == A_M2L ? 1u : 0u, act == A_P2P ? 1u : 0u};
-  uint32_t x[3], tot[3];
-  block_scan3(v, x, tot, sh);
-  if (!valid || act == A_NONE) return;
-  const size_t nb = gridDim.x;
-  const uint2 pr = fin[t];
-  if (act == A_M2L) {
-    const uint32_t wp = m2l0 + blk[nb + blockIdx.x] + x[1];
-    if (wp < cap_m2l)
-      m2l[wp] = pr;
-    else
-      atomicOr(&res->status, 2u);
-    return;
-  }
-  if (act == A_P2P) {
-    const uint32_t wp = p2p0 + blk[2 * nb + blockIdx.x] + x[2];
-    if (wp < cap_p2p)
-      p2p[wp] = pr;
-    else
-      atomicOr(&res->status, 4u);
-    return;
-  }
-  uint32_t wp = blk[blockIdx.x] + x[0];
-  if (wp + v[0] > cap_f) {
-    atomicOr(&res->status, 1u);
-    return;
-  }
-  if (act == A_SELF_SPLIT) {
-    const WalkRec ra = rec[pr.x];
-    for (int i = 0; i < 8; i++) {
-      if (ra.ch[i] < 0) continue;
-      for (int j = i; j < 8; j++)
-        if (ra.ch[j] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
+  __shared__ uint2 stage[WALK_STAGE];
+  if (st->status != 0u) return;
+  uint32_t b0, b1;
+  walk_slice(st->fsize[r & 1u], &b0, &b1);
+  // the scan checked that the whole round fits its buffers
+  uint32_t fo = blk[blockIdx.x];
+  uint32_t mo = st->m2l_base + blk[WALK_G + blockIdx.x];
+  uint32_t po = st->p2p_base + blk[2 * WALK_G + blockIdx.x];
+  for (uint32_t t0 = b0; t0 < b1; t0 += CW_T) {
+    const uint32_t t = t0 + threadIdx.x;
+    const bool valid = t < b1;
+    const uint32_t cd = valid ? (uint32_t)code[t] : 0u;
+    const int act = valid ? (int)(cd & 7u) : A_NONE;
+    const uint32_t v[3] = {cd >> 3, act == A_M2L ? 1u : 0u, act == A_P2P ? 1u : 0u};
+    uint32_t x[3], tot[3];
+    block_scan3(v, x, tot, sh);
+    const uint2 pr = valid ? fin[t] : make_uint2(0u, 0u);
+    if (act == A_M2L) m2l[mo + x[1]] = pr;
+    if (act == A_P2P) p2p[po + x[2]] = pr;
+    // child pairs: staged in LDS and written as one contiguous run when the
+    // batch's fit (always, unless self pairs of cells with many children
+    // crowd it), else written by each pair directly
+    const bool staged = tot[0] <= WALK_STAGE;
+    uint2* dst = staged ? stage : fout + fo;
+    uint32_t wp = x[0];
+    if (act == A_SELF_SPLIT) {
+      const WalkRec ra = rec[pr.x];
+      for (int i = 0; i < 8; i++) {
+        if (ra.ch[i] < 0) continue;
+        for (int j = i; j < 8; j++)
+          if (ra.ch[j] >= 0) dst[wp++] = make_uint2((uint32_t)ra.ch[i], (uint32_t)ra.ch[j]);
+      }
+    } else if (act == A_SPLIT_B) {
+      const WalkRec rb = rec[pr.y];
+      for (int j = 0; j < 8; j++)
+        if (rb.ch[j] >= 0) dst[wp++] = make_uint2(pr.x, (uint32_t)rb.ch[j]);
+    } else if (act == A_SPLIT_A) {
+      const WalkRec ra = rec[pr.x];
+      for (int i = 0; i < 8; i++)
+        if (ra.ch[i] >= 0) dst[wp++] = make_uint2((uint32_t)ra.ch[i], pr.y);
     }
-  } else if (act == A_SPLIT_B) {
-    const WalkRec rb = rec[pr.y];
-    for (int j = 0; j < 8; j++)
-      if (rb.ch[j] >= 0) fout[wp++] = make_uint2(pr.x, (uint32_t)rb.ch[j]);
-  } else {
-    const WalkRec ra = rec[pr.x];
-    for (int i = 0; i < 8; i++)
-      if (ra.ch[i] >= 0) fout[wp++] = make_uint2((uint32_t)ra.ch[i], pr.y);
+    if (staged) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < tot[0]; i += CW_T) fout[fo + i] = stage[i];
+      __syncthreads();
+    }
+    fo += tot[0];
+    mo += tot[1];
+    po += tot[2];
   }
 }
 
-uint32_t walk_blocks(uint32_t nf) { return (nf + CW_T - 1) / CW_T; }
-
-void launch_walk_decide(const uint2* fin, uint32_t nf, uint16_t* code, uint32_t* blk,
-                        const void* rec, const double* mass, MacParams mac, WalkResult* res,
-                        hipStream_t s) {
-  const uint32_t nb = walk_blocks(nf);
-  hipLaunchKernelGGL(k_walk_decide, dim3(nb), dim3(CW_T), 0, s, fin, nf, code, blk,
-                     (const WalkRec*)rec, mass, mac);
-  hipLaunchKernelGGL(k_walk_scan, dim3(1), dim3(1024), 0, s, blk, nb, res);
-}
-
-void launch_walk_emit(const uint2* fin, uint32_t nf, uint2* fout, uint32_t cap_f, uint2* m2l,
-                      uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, const uint16_t* code,
-                      const uint32_t* blk, uint32_t m2l0, uint32_t p2p0, WalkResult* res,
-                      const void* rec, hipStream_t s) {
-  hipLaunchKernelGGL(k_walk_emit, dim3(walk_blocks(nf)), dim3(CW_T), 0, s, fin, nf, fout, cap_f,
-                     m2l, cap_m2l, p2p, cap_p2p, code, blk, m2l0, p2p0, res, (const WalkRec*)rec);
+void launch_walk_round(const void* fin, void* fout, uint32_t r, WalkState* st, uint32_t cap_f,
+                       uint2* m2l, uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, uint16_t* code,
+                       uint32_t* blk, const void* rec, const double* mass, MacParams mac,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_walk_decide, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r, st, code,
+                     blk, (const WalkRec*)rec, mass, mac);
+  hipLaunchKernelGGL(k_walk_scan, dim3(1), dim3(1024), 0, s, blk, r, st, cap_f, cap_m2l, cap_p2p);
+  hipLaunchKernelGGL(k_walk_emit, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r, st,
+                     (uint2*)fout, m2l, p2p, code, blk, (const WalkRec*)rec);
 }
 
 // Row pointers of the target-sorted directed entries over all cells:
