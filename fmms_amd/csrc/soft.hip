@@ -116,9 +116,17 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
   double acc[NACC];
 #pragma unroll
   for (int k = 0; k < NACC; k++) acc[k] = 0.0;
-  for (uint32_t j = tptr[t] + lane; j < tptr[t + 1]; j += 64) {
-    const uint32_t src = (uint32_t)(skeys[j] & smask);
+  const uint32_t jend = tptr[t + 1];
+  // the lane's sources j, j + 64, j + 128, ... in order; the next one's key
+  // is loaded while this one is computed
+  uint32_t jn = tptr[t] + lane;
+  uint32_t src_n = jn < jend ? (uint32_t)(skeys[jn] & smask) : 0u;
+  for (uint32_t j = jn; j < jend; j += 64) {
+    const uint32_t src = src_n;
     const double4 sc = cpm[src];
+    const double qs = q[src];
+    jn = j + 64;
+    if (jn < jend) src_n = (uint32_t)(skeys[jn] & smask);
     const double rx = tx - sc.x, ry = ty - sc.y, rz = tz - sc.z;
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double s2 = r2 + soft2;
@@ -131,7 +139,7 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     const double l4 = 15.0 * soft2 * (3.0 * is7 - 7.0 * r2 * is9);     // Lap^2 K
     const double g4 = 15.0 * soft2 * (63.0 * r2 * is11 - 35.0 * is9);  // grad Lap^2 K / r
     const double ms = sc.w;
-    const double w = q[src] / 6.0;
+    const double w = qs / 6.0;
     // source side: w Lap K, its gradient, the traceless part of its Hessian
     const double gw = w * gk, hw = 0.5 * w * hk;
     acc[0] += w * lk;
