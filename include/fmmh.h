@@ -49,11 +49,16 @@ extern "C" {
 /* Precision forms */
 #define FMMH_PREC_FP64 0      /* plain FP64 everywhere (the drop-in) */
 #define FMMH_PREC_FP32BULK 1  /* mixed precision: P2P pair terms in FP32 on positions
-                                 relative to the target leaf's centre and M2L in FP32
-                                 on radius-scaled operands (per-cell scale, scaled
-                                 moments, dimensionless offsets), with FP64 tile and
-                                 chunk sums and FP64 final accumulation; the tree,
-                                 the upward pass, L2L and L2P stay FP64 */
+                                 relative to the target leaf's centre; for p <= 10
+                                 also P2M, M2M below the top three levels of each
+                                 system, M2L, L2L and L2P in FP32 on radius-scaled
+                                 operands (per-cell scale, scaled moments and locals,
+                                 dimensionless offsets); FP64 positions, tree, top-level
+                                 M2M, tile and chunk sums, softened-field trace terms,
+                                 and a compensated (TwoSum) final per-body near + far
+                                 sum. With a finite eps_mac the upward pass stays FP64
+                                 (the error MAC reads every cell's FP64 moments); for
+                                 p > 10 the far field runs the FP64 kernels. */
 
 /* Flags */
 #define FMMH_FLAG_TIMING 1u   /* record per-stage device times (hipEvents) */
