@@ -146,10 +146,12 @@ def stage_work(wc, p, precision):
     mass-weighted sum); L2P 10 dim(p) + 8 (dim(p) + 2 p^2) per body. Byte
     stages count the HBM bytes their algorithm must move: tree 336 per body
     (Morton key in 44, eight sort passes of 12-byte pairs read and written
-    192, gather to slot order 100), lists 16 bytes per directed key per radix
-    pass plus 8 to form it and 8 per pair read, the walk one 64-byte record
-    pair per decided pair, the M2L reduce its chunk partials in and locals
-    out."""
+    192, gather to slot order 100); lists, per directed key, 16 in the first
+    radix pass (its half of the pair read by the histogram and by the
+    scatter, the 8-byte key written), 24 in each later pass (read twice,
+    written once; ceil(target bits / 8) passes in all) and 8 for the row
+    pointers; the walk one 64-byte record pair per decided pair, the M2L
+    reduce its chunk partials in and locals out."""
     n, nc = wc["n"], wc["ncells"]
     dim = (p + 1) ** 2
     nf = (p + 1) * (p + 2) // 2
@@ -157,13 +159,13 @@ def stage_work(wc, p, precision):
     mixed = precision == "fp32bulk"
     f32 = "f32" if mixed else "f64"
     sbits = max(1, int(nc).bit_length())
-    passes = (2 * sbits + 7) // 8
+    passes = (sbits + 7) // 8
     keys = 2 * (m2l + p2p)
     return {
         "tree": ("hbm", 336.0 * n, None),
         "upward": ("valu", 14.0 * dim * n + 8.0 * t_full(p) * max(nc - 1, 0), "f64"),
         "walk": ("hbm", 128.0 * (m2l + p2p), None),
-        "lists": ("hbm", keys * (16.0 * passes + 8.0) + 8.0 * (m2l + p2p), None),
+        "lists": ("hbm", keys * (16.0 + 24.0 * (passes - 1) + 8.0), None),
         "m2l": ("valu", 8.0 * t_full(p) * 2 * m2l, f32),
         "m2l_reduce": ("hbm", (2.0 * m2l / 32 + nc) * nf * (8 if mixed else 16) + 16.0 * dim * nc,
                        None),
@@ -718,6 +720,7 @@ def main():
         log(f"[bench] {precision}: timing {args.steps} steps")
         elapsed, lat, stages = run_steps(fa, hip, ctx, n, d_pos, d_mass, d_acc, d_pot, args.steps)
         wc = ctx.work_counts()
+        wc["device_bytes"] = ctx.device_bytes()
         acc = d_acc.download()
         ctx.close()
         err = None
