@@ -262,6 +262,10 @@ __device__ __forceinline__ void l2p_trace_walk(const double* __restrict__ lap,
                                                const double* __restrict__ cy,
                                                const double* __restrict__ cz, uint32_t leaf,
                                                double bx, double by, double bz, double4& o) {
+  // the polynomial's constant factors as reciprocals: a multiply each
+  // instead of an FP64 division sequence per term, body and ancestor
+  constexpr double C3 = 1.0 / 3.0, C6 = 1.0 / 6.0, C10 = 1.0 / 10.0, C14 = 1.0 / 14.0,
+                   C28 = 1.0 / 28.0, C70 = 1.0 / 70.0, C280 = 1.0 / 280.0;
   for (int32_t A = (int32_t)leaf; A >= 0; A = parent[A]) {
     const double* t = lap + (size_t)SOFT_LAP_WORDS * A;
     const double dx = bx - cx[A], dy = by - cy[A], dz = bz - cz[A];
@@ -273,13 +277,13 @@ __device__ __forceinline__ void l2p_trace_walk(const double* __restrict__ lap,
     const double hz = t[8] * dx + t[9] * dy + t[6] * dz;
     const double dhd = dx * hx + dy * hy + dz * hz;
     const double tr = t[4] + t[5] + t[6];
-    o.w += d2 * L / 6.0 + d2 * dg / 10.0 + d2 * dhd / 28.0 - d2 * d2 * tr / 280.0;
-    o.x += dx * L / 3.0 + (2.0 * dx * dg + d2 * t[1]) / 10.0 + (dx * dhd + d2 * hx) / 14.0 -
-           d2 * dx * tr / 70.0;
-    o.y += dy * L / 3.0 + (2.0 * dy * dg + d2 * t[2]) / 10.0 + (dy * dhd + d2 * hy) / 14.0 -
-           d2 * dy * tr / 70.0;
-    o.z += dz * L / 3.0 + (2.0 * dz * dg + d2 * t[3]) / 10.0 + (dz * dhd + d2 * hz) / 14.0 -
-           d2 * dz * tr / 70.0;
+    o.w += d2 * L * C6 + d2 * dg * C10 + d2 * dhd * C28 - d2 * d2 * tr * C280;
+    o.x += dx * L * C3 + (2.0 * dx * dg + d2 * t[1]) * C10 + (dx * dhd + d2 * hx) * C14 -
+           d2 * dx * tr * C70;
+    o.y += dy * L * C3 + (2.0 * dy * dg + d2 * t[2]) * C10 + (dy * dhd + d2 * hy) * C14 -
+           d2 * dy * tr * C70;
+    o.z += dz * L * C3 + (2.0 * dz * dg + d2 * t[3]) * C10 + (dz * dhd + d2 * hz) * C14 -
+           d2 * dz * tr * C70;
   }
 }
 

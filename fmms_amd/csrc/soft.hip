@@ -25,12 +25,14 @@ namespace fmmh {
 constexpr int ST = 256;
 
 // q of each leaf from its bodies, one wave per leaf: the lanes form each
-// body's |x - z|^2 (64 bodies at a time, coalesced) and the sum runs in slot
-// order over the values read from the lanes that hold them.
+// body's |x - z|^2 (64 bodies at a time, coalesced) and stage them in LDS,
+// and lane 0 sums them in slot order.
 __global__ void __launch_bounds__(ST) k_trace_leaf(const uint32_t* __restrict__ leaf_ids,
                                                   uint32_t nleaves, DevCells c, DevBodies b,
                                                   double* __restrict__ q) {
-  const uint32_t i = blockIdx.x * (ST / 64) + (threadIdx.x >> 6);
+  __shared__ double chunk[ST / 64][2][64];  // per wave: m and |x - z|^2 of 64 bodies
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * (ST / 64) + wv;
   const uint32_t lane = threadIdx.x & 63;
   if (i >= nleaves) return;
   const uint32_t id = leaf_ids[i];
@@ -46,8 +48,13 @@ __global__ void __launch_bounds__(ST) k_trace_leaf(const uint32_t* __restrict__ 
       mk = b.m[k];
       d2 = dx * dx + dy * dy + dz * dz;
     }
+    chunk[wv][0][lane] = mk;
+    chunk[wv][1][lane] = d2;
+    __builtin_amdgcn_wave_barrier();
     const uint32_t cnt = min(64u, n - base);
-    for (uint32_t j = 0; j < cnt; j++) acc = fma(lane_read(mk, j), lane_read(d2, j), acc);
+    if (lane == 0)
+      for (uint32_t j = 0; j < cnt; j++) acc = fma(chunk[wv][0][j], chunk[wv][1][j], acc);
+    __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0) q[id] = acc;
 }
@@ -118,7 +125,8 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
   for (int k = 0; k < NACC; k++) acc[k] = 0.0;
   const uint32_t jend = tptr[t + 1];
   // the lane's sources j, j + 64, j + 128, ... in order; the next one's key
-  // is loaded while this one is computed
+  // is loaded while this one is computed (prefetching its record too costs
+  // a wave of occupancy)
   uint32_t jn = tptr[t] + lane;
   uint32_t src_n = jn < jend ? (uint32_t)(skeys[jn] & smask) : 0u;
   for (uint32_t j = jn; j < jend; j += 64) {
@@ -130,7 +138,7 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     const double rx = tx - sc.x, ry = ty - sc.y, rz = tz - sc.z;
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double s2 = r2 + soft2;
-    const double is = 1.0 / sqrt(s2);
+    const double is = rsqrt_pos(s2);  // s2 >= softening^2 > 0
     const double is2 = is * is;
     const double is5 = is2 * is2 * is, is7 = is5 * is2, is9 = is7 * is2, is11 = is9 * is2;
     const double lk = -3.0 * soft2 * is5;                              // Lap K
@@ -139,7 +147,7 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
     const double l4 = 15.0 * soft2 * (3.0 * is7 - 7.0 * r2 * is9);     // Lap^2 K
     const double g4 = 15.0 * soft2 * (63.0 * r2 * is11 - 35.0 * is9);  // grad Lap^2 K / r
     const double ms = sc.w;
-    const double w = qs / 6.0;
+    const double w = qs * (1.0 / 6.0);
     // source side: w Lap K, its gradient, the traceless part of its Hessian
     const double gw = w * gk, hw = 0.5 * w * hk;
     acc[0] += w * lk;

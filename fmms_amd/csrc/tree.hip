@@ -686,37 +686,44 @@ __device__ __forceinline__ uint32_t wave_farthest(DevBodies b, uint32_t f, uint3
 
 __global__ void __launch_bounds__(64 * GEO_WAVES) k_geometry_leaf(
     const uint32_t* __restrict__ leaf_ids, uint32_t nleaves, DevCells c, DevBodies b, int mode) {
-  const uint32_t li = blockIdx.x * GEO_WAVES + (threadIdx.x >> 6);
+  // per wave: one chunk of the leaf's bodies (mass, x, y, z) for the serial sums
+  __shared__ double chunk[GEO_WAVES][4][64];
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t li = blockIdx.x * GEO_WAVES + wv;
   const int lane = threadIdx.x & 63;
   if (li >= nleaves) return;
   const uint32_t id = leaf_ids[li];
   const uint32_t f = c.first[id], n = c.count[id];
-  // slot-ordered sums over chunks of 64 bodies held one per lane
-  double ms = 0.0, com[3] = {0.0, 0.0, 0.0};
+  // Slot-ordered sums over chunks of 64 bodies: the lanes stage the chunk in
+  // LDS, then lane 0 sums the masses (m * 1) and lanes 1-3 the mass-weighted
+  // coordinates, each in slot order (the wave's LDS operations run in program
+  // order; the wave barrier keeps the compiler from reordering them).
+  double sum = 0.0;  // lane 0: mass; lanes 1-3: sum of m x, m y, m z
   for (uint32_t base = 0; base < n; base += 64) {
     const uint32_t i = base + (uint32_t)lane;
     const bool have = i < n;
-    const double mi = have ? b.m[f + i] : 0.0;
     const uint32_t cnt = min(64u, n - base);
+    chunk[wv][0][lane] = have ? b.m[f + i] : 0.0;
     if (mode == 0) {  // com_of_slots, tree.rs:257-275
-      const double xi = have ? b.x[f + i] : 0.0, yi = have ? b.y[f + i] : 0.0,
-                   zi = have ? b.z[f + i] : 0.0;
-      for (uint32_t k = 0; k < cnt; k++) {
-        const double m = lane_read(mi, k);
-        ms += m;
-        com[0] += m * lane_read(xi, k);
-        com[1] += m * lane_read(yi, k);
-        com[2] += m * lane_read(zi, k);
-      }
-    } else {
-      for (uint32_t k = 0; k < cnt; k++) ms += lane_read(mi, k);
+      chunk[wv][1][lane] = have ? b.x[f + i] : 0.0;
+      chunk[wv][2][lane] = have ? b.y[f + i] : 0.0;
+      chunk[wv][3][lane] = have ? b.z[f + i] : 0.0;
     }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < (mode == 0 ? 4 : 1)) {
+      for (uint32_t k = 0; k < cnt; k++) {
+        const double m = chunk[wv][0][k];
+        sum += m * (lane == 0 ? 1.0 : chunk[wv][lane][k]);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
+  const double ms = lane_read(sum, 0);
   double cen[3];
   if (mode == 0) {
-    cen[0] = com[0] / ms;
-    cen[1] = com[1] / ms;
-    cen[2] = com[2] / ms;
+    cen[0] = lane_read(sum, 1) / ms;
+    cen[1] = lane_read(sum, 2) / ms;
+    cen[2] = lane_read(sum, 3) / ms;
   } else {  // Ritter sphere, tree.rs:290-389 (bodies have radius 0)
     const double MINP = 2.2250738585072014e-308;
     const double rad = 0.0;

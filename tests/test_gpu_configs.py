@@ -263,12 +263,19 @@ def test_ensemble_256_mixed_equals_standalone(fa):
         assert med <= MIXED_MED and p99 <= MIXED_P99 and mx <= MIXED_MAX
 
 
-def test_tree_2m_bit_exact(fa):
-    """N = 2,100,000: the key sort and the scans run over more than 256 tiles
-    (two-level scan, XCD-ordered sort tiles)."""
+def pair_codes(pairs):
+    """Pairs (a, b) as sorted uint64 codes a << 32 | b."""
+    a = np.asarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    return np.sort((a[:, 0] << np.uint64(32)) | a[:, 1])
+
+
+def test_tree_and_lists_2m_bit_exact(fa):
+    """N = 2,100,000: the key sort runs over 1,026 XCD-ordered sort tiles, and
+    the list sorts of the evaluation (tens of millions of directed keys) scan
+    digit histograms of more than 256 scan tiles (the two-level scan)."""
     n = 2100000
     pos, mass = O.plummer(n, seed=5)
-    say(f"[tree] N={n} bit-exact vs oracle")
+    say(f"[tree + lists] N={n} bit-exact vs oracle")
     orc = O.OracleFmm(pos, mass, p=4, eps=1e-3, n_leaf=24)
     g = fa.Fmm(pos, mass, p=4, eps=1e-3, n_leaf=24)
     tg, to = g.tree(), orc.tree()
@@ -276,6 +283,14 @@ def test_tree_2m_bit_exact(fa):
         assert np.array_equal(getattr(tg, k), getattr(to, k)), k
     for k in ("cen", "mass", "rho"):
         assert np.array_equal(bits(getattr(tg, k)), bits(getattr(to, k))), k
+    r = g.run(pos, mass)
+    a_ref, pot_ref = orc.run(pos, mass, nthreads=THREADS)
+    m2l, p2p = g.lists()
+    say(f"  {len(m2l)} M2L and {len(p2p)} P2P pairs")
+    assert len(m2l) > (1 << 22)  # > 256 scan tiles of the digit histograms
+    assert np.array_equal(pair_codes(m2l), pair_codes(orc.m2l_pairs())), "M2L list"
+    assert np.array_equal(pair_codes(p2p), pair_codes(orc.p2p_pairs())), "P2P list"
+    check_forces(r.accel, r.pot, a_ref, pot_ref)
 
 
 def test_host_samplers_match_oracle_on_gpu_host(fa):
