@@ -32,14 +32,12 @@ enum Stage {
   ST_M2L_REDUCE,
   ST_L2L,
   ST_L2P,
-  ST_P2P_PREP,  // source ranges of the directed entries
-  ST_P2P,       // the pair kernel alone
+  ST_P2P,  // the pair kernel alone
   ST_P2P_REDUCE,
   ST_COUNT
 };
-const char* kStageNames[ST_COUNT] = {"tree", "upward", "prepass",  "walk", "lists",
-                                     "m2l",  "m2l_reduce", "l2l", "l2p", "p2p_prep",
-                                     "p2p",  "p2p_reduce"};
+const char* kStageNames[ST_COUNT] = {"tree", "upward", "prepass",    "walk", "lists", "m2l",
+                                     "m2l_reduce", "l2l", "l2p", "p2p", "p2p_reduce"};
 
 struct Buf {
   void* p = nullptr;
@@ -86,7 +84,8 @@ struct fmmh_ctx {
   uint32_t* order = nullptr;
   uint32_t n_m2l = 0, n_p2p = 0, walk_rounds = 0;
   uint32_t walk_batch = 8;  // walk rounds launched per host check
-  uint64_t p2p_interactions = 0;
+  uint64_t p2p_interactions = 0;  // directed body-body pairs of the lists (on request)
+  bool p2p_counted = false;
   // timing
   hipEvent_t ev[ST_COUNT + 1] = {};
   bool ev_init = false;
@@ -94,7 +93,7 @@ struct fmmh_ctx {
   // buffers
   Buf in_pos, in_mass, out_acc, out_pot;
   Buf keys, keys_alt, vals, vals_alt;
-  Buf bx, by, bz, bm, bxyzm, prange;
+  Buf bx, by, bz, bm, bxyzm;
   Buf bbox_partial, bbox, flags;
   Buf chain, top, cstart;
   Buf bfs_first, bfs_count, bfs_parent, bfs_oct, bfs_level, bfs_dfs, nchild, child_off;
@@ -126,7 +125,7 @@ struct fmmh_ctx {
 
   fmmh_ctx() {
     Buf* all[] = {&in_pos, &in_mass, &out_acc, &out_pot, &keys, &keys_alt, &vals, &vals_alt,
-                  &bx, &by, &bz, &bm, &bxyzm, &prange, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
+                  &bx, &by, &bz, &bm, &bxyzm, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
@@ -514,9 +513,7 @@ struct fmmh_ctx {
     psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
     launch_csr_ptr(psorted, 2 * n_p2p, sbits, nc, (uint32_t*)pptr.p, s);
     pnnz = read1((const uint32_t*)pptr.p + nc);
-    ensure(p2p_work, (1 + p2p_count_scratch(n_p2p)) * 8);
-    launch_p2p_count((const uint2*)p2p.p, n_p2p, (const uint32_t*)c_count.p,
-                     (unsigned long long*)p2p_work.p, s);
+    p2p_counted = false;
     plan_p2p();
     if (p <= TEMPLATED_MAX_P) {
       const uint32_t ch = (uint32_t)tun.m2l_chunk;
@@ -541,7 +538,6 @@ struct fmmh_ctx {
       rot_tables();
     }
     ensure(far, n * sizeof(double4));
-    ensure(prange, (size_t)std::max<uint32_t>(pnnz, 1) * 8);
   }
 
   // P2P work items (near.hip): per leaf the item count, partial-sum slots and
@@ -631,18 +627,15 @@ struct fmmh_ctx {
       launch_l2p(p, (uint32_t)n, (uint32_t*)leaf_of_slot.p, c, b, loc, lap, (double4*)far.p, s);
     else
       launch_l2p_gen(p, (uint32_t*)leaf_ids.p, nleaves, c, b, loc, lap, (double4*)far.p, s);
-    mark(ST_P2P_PREP);
-    launch_p2p_ranges(psorted, pnnz, sbits, c, (uint2*)prange.p, s);
     const P2PPlan pl{(const uint4*)pl_items.p, p_nitems, (const uint4*)pl_reds.p, p_nsplit,
                      (double4*)pl_partial.p};
     const bool f32 = prm.precision == FMMH_PREC_FP32BULK;
     mark(ST_P2P);
     if (f32)
-      launch_p2p_f32(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order,
-                     acc, pot, s);
+      launch_p2p_f32(pl, c, b.xyzm, psorted, sbits, soft2, (const double4*)far.p, order, acc, pot,
+                     s);
     else
-      launch_p2p(pl, c, b.xyzm, (const uint2*)prange.p, soft2, (const double4*)far.p, order, acc,
-                 pot, s);
+      launch_p2p(pl, c, b.xyzm, psorted, sbits, soft2, (const double4*)far.p, order, acc, pot, s);
     mark(ST_P2P_REDUCE);
     launch_p2p_reduce(pl, c, (const double4*)far.p, order, acc, pot, f32, s);
     mark(ST_COUNT);
@@ -669,8 +662,8 @@ struct fmmh_ctx {
     mark(ST_LISTS);
     build_lists();
     far_near(acc, pot);
-    p2p_interactions = read1((const unsigned long long*)p2p_work.p);
     HCHECK(hipGetLastError());
+    HCHECK(hipStreamSynchronize(s));  // every call returns with its results in place
     evaluated = true;
     lists_ok = true;
     tree_evals = 1;
@@ -1334,6 +1327,19 @@ const char* fmmh_stage_name(int stage) {
 
 int fmmh_work_counts(const fmmh_ctx* ctx, uint64_t* out8) {
   if (!ctx) return FMMH_E_STATE;
+  if (ctx->lists_ok && !ctx->p2p_counted) {
+    // counted on request, not in every evaluation: the lists are unchanged
+    // until the next full evaluation
+    fmmh_ctx* c = const_cast<fmmh_ctx*>(ctx);
+    int rc = guarded(c, [&] {
+      c->ensure(c->p2p_work, (1 + p2p_count_scratch(c->n_p2p)) * 8);
+      launch_p2p_count((const uint2*)c->p2p.p, c->n_p2p, (const uint32_t*)c->c_count.p,
+                       (unsigned long long*)c->p2p_work.p, c->s);
+      c->p2p_interactions = c->read1((const unsigned long long*)c->p2p_work.p);
+      c->p2p_counted = true;
+    });
+    if (rc) return rc;
+  }
   out8[0] = ctx->n;
   out8[1] = ctx->nc;
   out8[2] = ctx->nleaves;

@@ -306,14 +306,13 @@ void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* p
                    const uint64_t* skeys, int sbits, const uint32_t* ccount, uint64_t item_work,
                    const uint32_t* ioff, const uint32_t* poff, const uint32_t* roff, uint4* items,
                    uint4* reds, hipStream_t s);
-void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
-                       hipStream_t s);
-void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
-                double soft2, const double4* far, const uint32_t* order, double* accel,
-                double* pot, hipStream_t s);
-void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
-                    double soft2, const double4* far, const uint32_t* order, double* accel,
-                    double* pot, hipStream_t s);
+// skeys: the target-sorted directed P2P entries (source in the low sbits)
+void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint64_t* skeys,
+                int sbits, double soft2, const double4* far, const uint32_t* order,
+                double* accel, double* pot, hipStream_t s);
+void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint64_t* skeys,
+                    int sbits, double soft2, const double4* far, const uint32_t* order,
+                    double* accel, double* pot, hipStream_t s);
 // Leaves cut into several items: their partial sums and the far field, added
 // in item order (compensated in the mixed form) into accel / pot.
 void launch_p2p_reduce(const P2PPlan& pl, DevCells c, const double4* far, const uint32_t* order,

@@ -191,6 +191,27 @@ __device__ __forceinline__ void acc2(double& s, double& c, double v) {
   s = t;
 }
 
+// Slot ranges of one segment of an item's source stream into LDS: rng[0] =
+// the leaf's own bodies when the stream starts with them (else empty), then
+// each directed entry's source leaf (first slot, count), the own leaf's entry
+// emptied. The barriers keep the previous pass's readers and this pass's
+// readers apart.
+__device__ __forceinline__ void stage_ranges(uint2* rng, const uint64_t* __restrict__ skeys,
+                                             uint64_t smask, const uint32_t* __restrict__ cfirst,
+                                             const uint32_t* __restrict__ ccount, uint32_t ft,
+                                             uint32_t nt_all, bool pre, uint32_t seg, uint32_t nr,
+                                             uint32_t lane) {
+  __syncthreads();
+  if (lane == 0) rng[0] = make_uint2(ft, pre ? nt_all : 0u);
+  for (uint32_t q = lane; q < nr; q += 64) {
+    const uint32_t src = (uint32_t)(skeys[seg + q] & smask);
+    uint2 g = make_uint2(cfirst[src], ccount[src]);
+    if (g.x == ft) g.y = 0;
+    rng[1 + q] = g;
+  }
+  __syncthreads();
+}
+
 // One wave per work item. The targets of the item's leaf are taken in groups
 // of 64, each group cut into chunks (chunk_plan); the item's source stream
 // (the leaf's own bodies first when the item holds them, each target
@@ -201,7 +222,8 @@ __device__ __forceinline__ void acc2(double& s, double& c, double v) {
 // FP32 and compensates the final sums (sub-lane partials, then the far field).
 template <bool F32>
 __global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
-                                           const uint2* __restrict__ prange,
+                                           const uint64_t* __restrict__ skeys,
+                                           uint64_t smask,
                                            const uint32_t* __restrict__ cfirst,
                                            const uint32_t* __restrict__ ccount,
                                            const double* __restrict__ ccx,
@@ -258,14 +280,9 @@ __global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
       const uint32_t seg = jb + sg * P2P_RMAX;
       const uint32_t nr = seg < je ? min(P2P_RMAX, je - seg) : 0u;
       const bool pre = own && sg == 0;  // stream starts with the own leaf
-      __syncthreads();
-      if (lane == 0) rng[0] = make_uint2(ft, pre ? nt_all : 0u);
-      for (uint32_t q = lane; q < nr; q += 64) {
-        uint2 g = prange[seg + q];
-        if (g.x == ft) g.y = 0;
-        rng[1 + q] = g;
-      }
-      __syncthreads();
+      // an item of one segment stages its ranges once, for every group
+      if (nseg > 1 || t0 == 0) stage_ranges(rng, skeys, smask, cfirst, ccount, ft, nt_all, pre,
+                                            seg, nr, lane);
       uint32_t rr = 0, off = 0, pos = 0;
       TileLoad cur = gather_tile(rng, nr + 1, rr, off, lane, body, pad);
       while (cur.fill > 0) {
@@ -351,18 +368,6 @@ __global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
       }
     }
   }
-}
-
-// prange[j] = (first, count) of the source leaf of directed entry j.
-__global__ void __launch_bounds__(256) k_p2p_ranges(const uint64_t* __restrict__ skeys,
-                                                   uint32_t nnz, uint64_t smask,
-                                                   const uint32_t* __restrict__ cfirst,
-                                                   const uint32_t* __restrict__ ccount,
-                                                   uint2* __restrict__ prange) {
-  uint32_t j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= nnz) return;
-  const uint32_t src = (uint32_t)(skeys[j] & smask);
-  prange[j] = make_uint2(cfirst[src], ccount[src]);
 }
 
 // ------------------------------------------------------- P2P work plan
@@ -569,19 +574,12 @@ void p2p_plan_emit(const uint32_t* leaf_ids, uint32_t nleaves, const uint32_t* p
                      nleaves, pptr, skeys, smask, ccount, item_work, ioff, poff, roff, items, reds);
 }
 
-void launch_p2p_ranges(const uint64_t* skeys, uint32_t nnz, int sbits, DevCells c, uint2* prange,
-                       hipStream_t s) {
-  if (nnz == 0) return;
-  const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_p2p_ranges, dim3((nnz + 255) / 256), dim3(256), 0, s, skeys, nnz, smask,
-                     c.first, c.count, prange);
-}
-
-void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
-                    double soft2, const double4* far, const uint32_t* order, double* accel,
-                    double* pot, hipStream_t s) {
+void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const uint64_t* skeys,
+                    int sbits, double soft2, const double4* far, const uint32_t* order,
+                    double* accel, double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
-  hipLaunchKernelGGL(k_p2p<true>, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  hipLaunchKernelGGL(k_p2p<true>, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
@@ -596,11 +594,12 @@ void launch_p2p_reduce(const P2PPlan& pl, DevCells c, const double4* far, const 
                        c.count, pl.partial, far, order, accel, pot);
 }
 
-void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint2* prange,
-                double soft2, const double4* far, const uint32_t* order, double* accel,
+void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint64_t* skeys,
+                int sbits, double soft2, const double4* far, const uint32_t* order, double* accel,
                 double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
-  hipLaunchKernelGGL(k_p2p<false>, dim3(pl.nitems), dim3(64), 0, s, pl.items, prange, c.first,
+  const uint64_t smask = (1ull << sbits) - 1ull;
+  hipLaunchKernelGGL(k_p2p<false>, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
