@@ -153,6 +153,9 @@ __device__ __forceinline__ void p2p_chunk(const typename P2PForm<F32>::T4* __res
 // idle targets. The plan weighs the padding against a fixed cost per chunk
 // and tile (its sums and loop set-up), in VALU issues.
 constexpr int P2P_NQ = 3;
+#ifndef FMMH_P2P_WAVES_F32
+#define FMMH_P2P_WAVES_F32 5  // waves per SIMD the mixed-form P2P kernel is built for
+#endif
 
 __device__ __forceinline__ uint32_t pow2ceil(uint32_t x) {
   return x <= 1u ? 1u : 1u << (32 - __clz(x - 1u));
@@ -220,20 +223,18 @@ __device__ __forceinline__ void stage_ranges(uint2* rng, const uint64_t* __restr
 // every chunk runs on every tile. The next tile's global loads are issued
 // before the current one is computed. The mixed form (F32) sums each tile in
 // FP32 and compensates the final sums (sub-lane partials, then the far field).
+#define FMMH_P2P_PARAMS                                                                     \
+  const uint4 *__restrict__ items, const uint64_t *__restrict__ skeys, uint64_t smask,       \
+      const uint32_t *__restrict__ cfirst, const uint32_t *__restrict__ ccount,               \
+      const double *__restrict__ ccx, const double *__restrict__ ccy,                         \
+      const double *__restrict__ ccz, const double4 *__restrict__ body, double soft2,         \
+      const double4 *__restrict__ far, const uint32_t *__restrict__ order,                    \
+      double *__restrict__ accel, double *__restrict__ pot, double4 *__restrict__ partial
+#define FMMH_P2P_ARGS \
+  items, skeys, smask, cfirst, ccount, ccx, ccy, ccz, body, soft2, far, order, accel, pot, partial
+
 template <bool F32>
-__global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
-                                           const uint64_t* __restrict__ skeys,
-                                           uint64_t smask,
-                                           const uint32_t* __restrict__ cfirst,
-                                           const uint32_t* __restrict__ ccount,
-                                           const double* __restrict__ ccx,
-                                           const double* __restrict__ ccy,
-                                           const double* __restrict__ ccz,
-                                           const double4* __restrict__ body, double soft2,
-                                           const double4* __restrict__ far,
-                                           const uint32_t* __restrict__ order,
-                                           double* __restrict__ accel, double* __restrict__ pot,
-                                           double4* __restrict__ partial) {
+__device__ __forceinline__ void p2p_item(FMMH_P2P_PARAMS) {
   using F = P2PForm<F32>;
   using T = typename F::T;
   __shared__ typename F::T4 tile[64];
@@ -368,6 +369,19 @@ __global__ void __launch_bounds__(64) k_p2p(const uint4* __restrict__ items,
       }
     }
   }
+}
+
+// The two forms as kernels of their own, each with the occupancy that
+// measured fastest: the mixed form at five waves per SIMD (at most 102 VGPRs:
+// the unrolled tile loop keeps fewer reads in flight, and the extra wave
+// hides more of the tile staging), FP64 at the compiler's choice.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FMMH_P2P_WAVES_F32)))
+k_p2p_mixed(FMMH_P2P_PARAMS) {
+  p2p_item<true>(FMMH_P2P_ARGS);
+}
+
+__global__ void __launch_bounds__(64) k_p2p_fp64(FMMH_P2P_PARAMS) {
+  p2p_item<false>(FMMH_P2P_ARGS);
 }
 
 // ------------------------------------------------------- P2P work plan
@@ -579,7 +593,7 @@ void launch_p2p_f32(const P2PPlan& pl, DevCells c, const double4* body, const ui
                     double* accel, double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_p2p<true>, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
+  hipLaunchKernelGGL(k_p2p_mixed, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
@@ -599,7 +613,7 @@ void launch_p2p(const P2PPlan& pl, DevCells c, const double4* body, const uint64
                 double* pot, hipStream_t s) {
   if (pl.nitems == 0) return;
   const uint64_t smask = (1ull << sbits) - 1ull;
-  hipLaunchKernelGGL(k_p2p<false>, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
+  hipLaunchKernelGGL(k_p2p_fp64, dim3(pl.nitems), dim3(64), 0, s, pl.items, skeys, smask, c.first,
                      c.count, c.cx, c.cy, c.cz, body, soft2, far, order, accel, pot, pl.partial);
 }
 
