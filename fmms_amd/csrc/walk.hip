@@ -91,6 +91,7 @@ __device__ __forceinline__ double error_estimate(uint32_t a, double rho_b, doubl
 // r2*theta_max^2 <= (rhoA+rhoB)^2 rejects; theta_max = 1 multiplies by 1.
 // Softening guard extension: eps^2 (rhoA+rhoB)^3 > tau r^5 rejects (the
 // softened far field's leading residual relative to the pair's field).
+template <bool ERR>
 __device__ __forceinline__ bool accept(uint32_t a, const WalkRec& ra, uint32_t b,
                                        const WalkRec& rb, double r2, const double* __restrict__ mass,
                                        const MacParams& mp) {
@@ -99,7 +100,7 @@ __device__ __forceinline__ bool accept(uint32_t a, const WalkRec& ra, uint32_t b
   if (mp.soft_mac > 0.0 && mp.soft2 > 0.0 &&
       mp.soft2 * (sum_rho * sum_rho * sum_rho) > mp.soft_mac * (r2 * r2 * sqrt(r2)))
     return false;
-  if (!mp.use_err) return true;
+  if (!ERR) return true;
   double r = sqrt(r2);
   if (r == 0.0) return false;
   const double ma = mass[a];
@@ -117,7 +118,10 @@ __device__ __forceinline__ uint32_t nchild(const WalkRec& r) {
   return k;
 }
 
-// walk.rs:94-190 decision of one frontier pair; nch = its child pairs.
+// walk.rs:94-190 decision of one frontier pair; nch = its child pairs. ERR:
+// the error-controlled MAC (mp.use_err); the theta-only walk is compiled
+// without it (no scratch for the error estimate's power table).
+template <bool ERR>
 __device__ __forceinline__ int decide(uint32_t a, uint32_t b, const WalkRec& ra,
                                       const WalkRec& rb, const double* __restrict__ mass,
                                       const MacParams& mp, uint32_t* nch) {
@@ -133,7 +137,8 @@ __device__ __forceinline__ int decide(uint32_t a, uint32_t b, const WalkRec& ra,
   double r2ab = dx * dx + dy * dy + dz * dz;
   double ex = ra.cx - rb.cx, ey = ra.cy - rb.cy, ez = ra.cz - rb.cz;
   double r2ba = ex * ex + ey * ey + ez * ez;
-  if (accept(a, ra, b, rb, r2ab, mass, mp) && accept(b, rb, a, ra, r2ba, mass, mp)) return A_M2L;
+  if (accept<ERR>(a, ra, b, rb, r2ab, mass, mp) && accept<ERR>(b, rb, a, ra, r2ba, mass, mp))
+    return A_M2L;
   if (la && lb) return A_P2P;
   bool split_b;
   if (la)
@@ -207,6 +212,7 @@ __device__ __forceinline__ void walk_slice(uint32_t nf, uint32_t* b0, uint32_t* 
 // plus its rank in the block. The lists thus come out in one fixed order,
 // and a system's pairs keep their relative order whether it is walked alone
 // or in an ensemble.
+template <bool ERR>
 __global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ fin, uint32_t r,
                                                     const WalkState* __restrict__ st,
                                                     uint16_t* __restrict__ code,
@@ -222,7 +228,7 @@ __global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ 
     const uint2 pr = fin[t];
     const WalkRec ra = rec[pr.x], rb = rec[pr.y];
     uint32_t nch;
-    const int act = decide(pr.x, pr.y, ra, rb, mass, mp, &nch);
+    const int act = decide<ERR>(pr.x, pr.y, ra, rb, mass, mp, &nch);
     code[t] = (uint16_t)(act | (nch << 3));
     v[0] += nch;
     v[1] += act == A_M2L ? 1u : 0u;
@@ -355,8 +361,12 @@ void launch_walk_round(const void* fin, void* fout, uint32_t r, WalkState* st, u
                        uint2* m2l, uint32_t cap_m2l, uint2* p2p, uint32_t cap_p2p, uint16_t* code,
                        uint32_t* blk, const void* rec, const double* mass, MacParams mac,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_walk_decide, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r, st, code,
-                     blk, (const WalkRec*)rec, mass, mac);
+  if (mac.use_err)
+    hipLaunchKernelGGL(k_walk_decide<true>, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r,
+                       st, code, blk, (const WalkRec*)rec, mass, mac);
+  else
+    hipLaunchKernelGGL(k_walk_decide<false>, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r,
+                       st, code, blk, (const WalkRec*)rec, mass, mac);
   hipLaunchKernelGGL(k_walk_scan, dim3(1), dim3(1024), 0, s, blk, r, st, cap_f, cap_m2l, cap_p2p);
   hipLaunchKernelGGL(k_walk_emit, dim3(WALK_G), dim3(CW_T), 0, s, (const uint2*)fin, r, st,
                      (uint2*)fout, m2l, p2p, code, blk, (const WalkRec*)rec);
