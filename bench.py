@@ -664,9 +664,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build runs the force evaluation on a single "
-              "MI355X and has no multi-GPU path (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the force evaluation runs on a single MI355X and "
+              "has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import fmms_amd as fa
