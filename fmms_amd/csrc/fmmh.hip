@@ -108,7 +108,7 @@ struct fmmh_ctx {
   Buf scan_tmp, sort_tmp;
   Buf moments, locals, pows, fvals, fmin, fact, binom;
   Buf front_a, front_b, m2l, p2p, ctr;
-  Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr;
+  Buf mkeys, mkeys_alt, tptr, pkeys, pkeys_alt, pptr, list_tot;
   Buf nch, choff, chunks, part;
   Buf far, p2p_work, walkrec;
   Buf walk_code, walk_blk;  // the walk's per-pair decisions, per-block counts
@@ -132,7 +132,7 @@ struct fmmh_ctx {
                   &leaf_flags, &leaf_off, &leaf_ids, &leaf_of_slot, &int_pos, &int_cells, &lvl_dev, &split_bnd, &leaf_bc, &lbc_keys, &lbc_keys_alt, &cpm,
                   &scan_tmp, &sort_tmp,
                   &moments, &locals, &pows, &fvals, &fmin, &fact, &binom, &front_a, &front_b,
-                  &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr,
+                  &m2l, &p2p, &ctr, &mkeys, &mkeys_alt, &tptr, &pkeys, &pkeys_alt, &pptr, &list_tot,
                   &nch, &choff, &chunks, &part, &far, &p2p_work, &walkrec, &walk_code, &walk_blk, &pl_n, &pl_items,
                   &pl_reds, &pl_partial, &m2l_lam, &m2l_m32, &loc32, &rt_sw, &rt_swt, &rt_din, &rt_dout, &rt_fin, &rt_fout,
                   &rt_fact, &direct_ids, &direct_out, &direct_part, &lf_part, &lf_state, &lf_flag, &ctrace, &clap,
@@ -413,7 +413,7 @@ struct fmmh_ctx {
   bool lists_ok = false;
   const uint64_t* msorted = nullptr;
   const uint64_t* psorted = nullptr;
-  uint32_t pnnz = 0, nchunks = 0;
+  uint32_t nchunks = 0;
   uint32_t tree_evals = 0;  // evaluations run on the current tree and lists
   bool capturing = false;
 
@@ -512,9 +512,13 @@ struct fmmh_ctx {
                              (uint64_t*)pkeys_alt.p, (uint32_t*)sort_tmp.p, s);
     psorted = (const uint64_t*)(fl ? pkeys_alt.p : pkeys.p);
     launch_csr_ptr(psorted, 2 * n_p2p, sbits, nc, (uint32_t*)pptr.p, s);
-    pnnz = read1((const uint32_t*)pptr.p + nc);
     p2p_counted = false;
-    plan_p2p();
+    // the P2P plan's three totals and the M2L chunk count, copied into one
+    // small buffer on the device and read back together
+    ensure(list_tot, 16);
+    uint32_t* dtot = (uint32_t*)list_tot.p;
+    HCHECK(hipMemsetAsync(dtot, 0, 16, s));
+    plan_p2p_count(dtot);
     if (p <= TEMPLATED_MAX_P) {
       const uint32_t ch = (uint32_t)tun.m2l_chunk;
       size_t maxck = m2l_max_chunks(2 * n_m2l, nc, ch);
@@ -527,7 +531,14 @@ struct fmmh_ctx {
                        (uint32_t*)scan_tmp.p, chunks.p, s, &tot);
       // choff[nc] = total chunks (the reduce reads choff[t+1])
       HCHECK(hipMemcpyAsync((uint32_t*)choff.p + nc, tot, 4, hipMemcpyDeviceToDevice, s));
-      nchunks = read1(tot);
+      HCHECK(hipMemcpyAsync(dtot + 3, tot, 4, hipMemcpyDeviceToDevice, s));
+    }
+    const uint4 t = read1((const uint4*)dtot);
+    p_nitems = t.x;
+    p_nsplit = t.z;
+    plan_p2p_emit(t.y);
+    if (p <= TEMPLATED_MAX_P) {
+      nchunks = t.w;
       ensure(part, m2l_part_bytes(p, std::max<size_t>(nchunks, 1)));
       if (prm.precision == FMMH_PREC_FP32BULK) {
         ensure(m2l_lam, (size_t)nc * sizeof(double));
@@ -541,32 +552,36 @@ struct fmmh_ctx {
   }
 
   // P2P work items (near.hip): per leaf the item count, partial-sum slots and
-  // split flag, scanned in place into offsets, then the items themselves.
-  void plan_p2p() {
+  // split flag, scanned in place into offsets (the three totals to
+  // tot[0..2]), then, once the totals are on the host, the items themselves.
+  void plan_p2p_count(uint32_t* tot) {
     p_nitems = p_nsplit = 0;
     if (nleaves == 0) return;
     const size_t w = (size_t)nleaves + 1;
     ensure(pl_n, 3 * w * 4);
     uint32_t* nitem = (uint32_t*)pl_n.p;
-    uint32_t* npart = nitem + w;
-    uint32_t* nsplit = npart + w;
-    const uint32_t* lids = (const uint32_t*)leaf_ids.p;
-    const uint32_t* ccount = (const uint32_t*)c_count.p;
-    p2p_plan_count(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount,
-                   tun.p2p_item_work, nitem, npart, nsplit, s);
+    p2p_plan_count((const uint32_t*)leaf_ids.p, nleaves, (const uint32_t*)pptr.p, psorted, sbits,
+                   (const uint32_t*)c_count.p, tun.p2p_item_work, nitem, nitem + w, nitem + 2 * w,
+                   s);
     ensure_scan(w);
-    uint32_t* tot = nullptr;
-    scan_exclusive_u32(nitem, nitem, w, (uint32_t*)scan_tmp.p, s, &tot);
-    p_nitems = read1(tot);
-    scan_exclusive_u32(npart, npart, w, (uint32_t*)scan_tmp.p, s, &tot);
-    const uint32_t nslots = read1(tot);
-    scan_exclusive_u32(nsplit, nsplit, w, (uint32_t*)scan_tmp.p, s, &tot);
-    p_nsplit = read1(tot);
+    for (int k = 0; k < 3; k++) {
+      uint32_t* col = nitem + k * w;
+      uint32_t* t = nullptr;
+      scan_exclusive_u32(col, col, w, (uint32_t*)scan_tmp.p, s, &t);
+      HCHECK(hipMemcpyAsync(tot + k, t, 4, hipMemcpyDeviceToDevice, s));
+    }
+  }
+
+  void plan_p2p_emit(uint32_t nslots) {
+    if (nleaves == 0) return;
+    const size_t w = (size_t)nleaves + 1;
+    uint32_t* nitem = (uint32_t*)pl_n.p;
     ensure(pl_items, (size_t)std::max<uint32_t>(p_nitems, 1) * sizeof(uint4));
     ensure(pl_reds, (size_t)std::max<uint32_t>(p_nsplit, 1) * sizeof(uint4));
     ensure(pl_partial, (size_t)std::max<uint32_t>(nslots, 1) * sizeof(double4));
-    p2p_plan_emit(lids, nleaves, (const uint32_t*)pptr.p, psorted, sbits, ccount,
-                  tun.p2p_item_work, nitem, npart, nsplit, (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
+    p2p_plan_emit((const uint32_t*)leaf_ids.p, nleaves, (const uint32_t*)pptr.p, psorted, sbits,
+                  (const uint32_t*)c_count.p, tun.p2p_item_work, nitem, nitem + w, nitem + 2 * w,
+                  (uint4*)pl_items.p, (uint4*)pl_reds.p, s);
   }
 
   // The mixed form's FP32 M2L, L2L and L2P (templated orders only).
