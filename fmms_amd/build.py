@@ -81,6 +81,24 @@ def build(verbose=False):
     return LIB
 
 
+# Measurement programs under tests/perf (not part of the library): source ->
+# executable next to it.
+PROBES = {os.path.join(ROOT, "tests", "perf", "rot_products.hip"):
+          os.path.join(ROOT, "tests", "perf", "rot_products")}
+
+
+def build_probes(verbose=False):
+    for src, exe in PROBES.items():
+        cmd = [HIPCC, *COMMON, "-ffp-contract=fast", "-o", exe + ".tmp", src]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print("built", exe)
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_probes(verbose=True)
     sys.exit(0)
