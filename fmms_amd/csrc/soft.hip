@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(256) k_m2l_trace(uint32_t nc, const uint32_t* 
                                                   const double* __restrict__ q,
                                                   double soft2, int D, cplx* __restrict__ locals,
                                                   double* __restrict__ lap) {
-  const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // neighbouring targets share sources: each XCD takes a contiguous range
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   if (t >= nc) return;
   const double4 tc = cpm[t];

@@ -192,11 +192,14 @@ __device__ __forceinline__ void block_scan3(const uint32_t v[3], uint32_t x[3], 
   __syncthreads();
 }
 
-// The round's frontier slice of this block: WALK_G blocks, each a
-// contiguous run of ceil(nf / WALK_G) pairs, in frontier order.
-__device__ __forceinline__ void walk_slice(uint32_t nf, uint32_t* b0, uint32_t* b1) {
+// The round's frontier slice of logical block lb: WALK_G blocks, each a
+// contiguous run of ceil(nf / WALK_G) pairs, in frontier order. Logical
+// blocks are spread so that each XCD takes a contiguous range of slices
+// (neighbouring pairs share cells, whose records then share one L2).
+__device__ __forceinline__ uint32_t walk_block() { return xcd_block(blockIdx.x, gridDim.x); }
+__device__ __forceinline__ void walk_slice(uint32_t nf, uint32_t lb, uint32_t* b0, uint32_t* b1) {
   const uint32_t chunk = (nf + WALK_G - 1) / WALK_G;
-  *b0 = min(blockIdx.x * chunk, nf);
+  *b0 = min(lb * chunk, nf);
   *b1 = min(*b0 + chunk, nf);
 }
 
@@ -221,8 +224,9 @@ __global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ 
                                                     const double* __restrict__ mass, MacParams mp) {
   __shared__ uint32_t sh[CW_T / 64][3];
   if (st->status != 0u) return;
+  const uint32_t lb = walk_block();
   uint32_t b0, b1;
-  walk_slice(st->fsize[r & 1u], &b0, &b1);
+  walk_slice(st->fsize[r & 1u], lb, &b0, &b1);
   uint32_t v[3] = {0u, 0u, 0u};  // child pairs, M2L, P2P
   for (uint32_t t = b0 + threadIdx.x; t < b1; t += CW_T) {
     const uint2 pr = fin[t];
@@ -235,7 +239,7 @@ __global__ void __launch_bounds__(CW_T) k_walk_decide(const uint2* __restrict__ 
     v[2] += act == A_P2P ? 1u : 0u;
   }
   block_sum3(v, sh);
-  if (threadIdx.x < 3) blk[WALK_G * threadIdx.x + blockIdx.x] = v[threadIdx.x];
+  if (threadIdx.x < 3) blk[WALK_G * threadIdx.x + lb] = v[threadIdx.x];
 }
 
 // One block of 1024 threads: exclusive scans of the three count columns
@@ -307,12 +311,13 @@ __global__ void __launch_bounds__(CW_T) k_walk_emit(const uint2* __restrict__ fi
   __shared__ uint32_t sh[CW_T / 64][3];
   __shared__ uint2 stage[WALK_STAGE];
   if (st->status != 0u) return;
+  const uint32_t lb = walk_block();
   uint32_t b0, b1;
-  walk_slice(st->fsize[r & 1u], &b0, &b1);
+  walk_slice(st->fsize[r & 1u], lb, &b0, &b1);
   // the scan checked that the whole round fits its buffers
-  uint32_t fo = blk[blockIdx.x];
-  uint32_t mo = st->m2l_base + blk[WALK_G + blockIdx.x];
-  uint32_t po = st->p2p_base + blk[2 * WALK_G + blockIdx.x];
+  uint32_t fo = blk[lb];
+  uint32_t mo = st->m2l_base + blk[WALK_G + lb];
+  uint32_t po = st->p2p_base + blk[2 * WALK_G + lb];
   for (uint32_t t0 = b0; t0 < b1; t0 += CW_T) {
     const uint32_t t = t0 + threadIdx.x;
     const bool valid = t < b1;
