@@ -94,6 +94,7 @@ struct fmmh_ctx {
   Buf in_pos, in_mass, out_acc, out_pot;
   Buf keys, keys_alt, vals, vals_alt;
   Buf bx, by, bz, bm, bxyzm;
+  Buf bpack;  // (x, y, z, m) in id order, written with the keys for the gather
   Buf bbox_partial, bbox, flags;
   Buf chain, top, cstart;
   Buf bfs_first, bfs_count, bfs_parent, bfs_oct, bfs_level, bfs_dfs, nchild, child_off;
@@ -125,7 +126,7 @@ struct fmmh_ctx {
 
   fmmh_ctx() {
     Buf* all[] = {&in_pos, &in_mass, &out_acc, &out_pot, &keys, &keys_alt, &vals, &vals_alt,
-                  &bx, &by, &bz, &bm, &bxyzm, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
+                  &bx, &by, &bz, &bm, &bxyzm, &bpack, &bbox_partial, &bbox, &flags, &chain, &top, &cstart,
                   &bfs_first, &bfs_count, &bfs_parent, &bfs_oct, &bfs_level, &bfs_dfs, &nchild,
                   &child_off, &c_first, &c_count, &c_children, &c_parent, &c_leaf, &c_level,
                   &c_cx, &c_cy, &c_cz, &c_mass, &c_rho, &c_rope, &c_last, &lvl_cells,
@@ -257,6 +258,7 @@ struct fmmh_ctx {
     ensure(bz, n * 8);
     ensure(bm, n * 8);
     ensure(bxyzm, n * 32);
+    ensure(bpack, n * 32);
     ensure(bbox_partial, 6 * 1024 * 8);
     ensure(bbox, nsys * sizeof(BBox));
     ensure(flags, 64);
@@ -271,7 +273,7 @@ struct fmmh_ctx {
     if (nsys == 1) {
       launch_bbox(pos, n, (double*)bbox_partial.p, (BBox*)bbox.p, s);
       launch_keys(pos, mass, n, (const BBox*)bbox.p, (uint64_t*)keys.p, (uint32_t*)vals.p,
-                  (uint32_t*)flags.p, s);
+                  (uint32_t*)flags.p, (double4*)bpack.p, s);
     } else {
       off = ensure<uint64_t>(d_sysoff, (nsys + 1) * 8);
       uint64_t* h = (uint64_t*)host_stage((nsys + 1) * 8);
@@ -279,7 +281,7 @@ struct fmmh_ctx {
       HCHECK(hipMemcpyAsync(off, h, (nsys + 1) * 8, hipMemcpyHostToDevice, s));
       launch_bbox_sys(pos, off, nsys, (BBox*)bbox.p, s);
       launch_keys_sys(pos, mass, n, off, nsys, vlev, (const BBox*)bbox.p, (uint64_t*)keys.p,
-                      (uint32_t*)vals.p, (uint32_t*)flags.p, s);
+                      (uint32_t*)vals.p, (uint32_t*)flags.p, (double4*)bpack.p, s);
     }
     int flip = radix_sort_u64((uint64_t*)keys.p, (uint32_t*)vals.p, (uint64_t*)keys_alt.p,
                               (uint32_t*)vals_alt.p, n, 0, 63, (uint32_t*)sort_tmp.p, s);
@@ -298,7 +300,7 @@ struct fmmh_ctx {
         order = v2;
       }
     }
-    launch_gather(pos, mass, order, n, bodies(), s);
+    launch_gather_packed((const double4*)bpack.p, order, n, bodies(), s);
     // level-by-level octant splits
     HCHECK(hipMemsetAsync(chain.p, 0, n * 4, s));
     HCHECK(hipMemsetAsync(top.p, 0, n, s));

@@ -66,15 +66,18 @@ void launch_bbox_sys(const double* pos, const uint64_t* off, uint32_t nsys, BBox
                      hipStream_t s);
 void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,
                      uint32_t nsys, int vlev, const BBox* bbs, uint64_t* keys, uint32_t* vals,
-                     uint32_t* flags, hipStream_t s);
+                     uint32_t* flags, double4* pk, hipStream_t s);
 // Ensemble keys on bodies sorted by standalone key: system prefix over the
 // key's top 63 - 3 vlev bits (then sorted stably on the prefix bits).
 void launch_keys_prefix(uint64_t* keys, const uint32_t* ids, size_t n, const uint64_t* off,
                         uint32_t nsys, int vlev, hipStream_t s);
+// Both keys kernels also write pk[i] = (x, y, z, m) of body i (id order).
 void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb,
-                 uint64_t* keys, uint32_t* vals, uint32_t* flags, hipStream_t s);
+                 uint64_t* keys, uint32_t* vals, uint32_t* flags, double4* pk, hipStream_t s);
 void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
                    DevBodies b, hipStream_t s);
+void launch_gather_packed(const double4* pk, const uint32_t* order, size_t n, DevBodies b,
+                          hipStream_t s);
 void launch_tree_init(DevBfs bfs, size_t n, uint32_t* chain, uint8_t* top, hipStream_t s);
 void launch_split_count(DevBfs bfs, uint32_t s0, uint32_t cnt, const uint64_t* keys,
                         uint32_t n_leaf, int vlev, uint32_t* nchild, uint32_t* bounds,

@@ -169,7 +169,8 @@ __global__ void __launch_bounds__(TB) k_keys(const double* __restrict__ pos,
                                             const BBox* __restrict__ bb,
                                             uint64_t* __restrict__ keys,
                                             uint32_t* __restrict__ vals,
-                                            uint32_t* __restrict__ flags) {
+                                            uint32_t* __restrict__ flags,
+                                            double4* __restrict__ pk) {
   size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
   if (i >= n) return;
   const double lo0 = bb->lo[0], inv = bb->inv_side;
@@ -179,6 +180,7 @@ __global__ void __launch_bounds__(TB) k_keys(const double* __restrict__ pos,
   key |= dilate21(quantise(x, lo0, inv)) << 2;
   keys[i] = key;
   vals[i] = (uint32_t)i;
+  pk[i] = make_double4(x, y, z, m);
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
 }
 
@@ -204,7 +206,8 @@ __global__ void __launch_bounds__(TB) k_keys_sys(const double* __restrict__ pos,
                                                 int vlev, const BBox* __restrict__ bbs,
                                                 uint64_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals,
-                                                uint32_t* __restrict__ flags) {
+                                                uint32_t* __restrict__ flags,
+                                                double4* __restrict__ pk) {
   size_t i = (size_t)blockIdx.x * TB + threadIdx.x;
   if (i >= n) return;
   const BBox* bb = bbs + system_of(off, nsys, i);
@@ -215,6 +218,7 @@ __global__ void __launch_bounds__(TB) k_keys_sys(const double* __restrict__ pos,
   key |= dilate21(quantise(x, lo0, inv)) << 2;
   keys[i] = key;
   vals[i] = (uint32_t)i;
+  pk[i] = make_double4(x, y, z, m);
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) atomicOr(flags, 1u);
 }
 
@@ -241,15 +245,15 @@ void launch_keys_prefix(uint64_t* keys, const uint32_t* ids, size_t n, const uin
 
 void launch_keys_sys(const double* pos, const double* mass, size_t n, const uint64_t* off,
                      uint32_t nsys, int vlev, const BBox* bbs, uint64_t* keys, uint32_t* vals,
-                     uint32_t* flags, hipStream_t s) {
+                     uint32_t* flags, double4* pk, hipStream_t s) {
   hipLaunchKernelGGL(k_keys_sys, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, n, off, nsys,
-                     vlev, bbs, keys, vals, flags);
+                     vlev, bbs, keys, vals, flags, pk);
 }
 
 void launch_keys(const double* pos, const double* mass, size_t n, const BBox* bb, uint64_t* keys,
-                 uint32_t* vals, uint32_t* flags, hipStream_t s) {
+                 uint32_t* vals, uint32_t* flags, double4* pk, hipStream_t s) {
   hipLaunchKernelGGL(k_keys, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, n, bb, keys, vals,
-                     flags);
+                     flags, pk);
 }
 
 // Bodies::from_parts (bodies.rs:27-47): id order AoS -> slot order SoA.
@@ -271,6 +275,28 @@ __global__ void __launch_bounds__(TB) k_gather(const double* __restrict__ pos,
 void launch_gather(const double* pos, const double* mass, const uint32_t* order, size_t n,
                    DevBodies b, hipStream_t s) {
   hipLaunchKernelGGL(k_gather, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pos, mass, order, n, b);
+}
+
+// The same gather from the keys kernel's packed copy (x, y, z, m in id
+// order, 32 bytes per body): one aligned 32-byte read per body instead of
+// the position triple and the mass from two arrays.
+__global__ void __launch_bounds__(TB) k_gather_packed(const double4* __restrict__ pk,
+                                                     const uint32_t* __restrict__ order, size_t n,
+                                                     DevBodies b) {
+  size_t k = (size_t)blockIdx.x * TB + threadIdx.x;
+  if (k >= n) return;
+  // the clamp keeps a failed sort from reading outside the inputs
+  const double4 v = pk[min((size_t)order[k], n - 1)];
+  b.x[k] = v.x;
+  b.y[k] = v.y;
+  b.z[k] = v.z;
+  b.m[k] = v.w;
+  b.xyzm[k] = v;
+}
+
+void launch_gather_packed(const double4* pk, const uint32_t* order, size_t n, DevBodies b,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_packed, dim3((n + TB - 1) / TB), dim3(TB), 0, s, pk, order, n, b);
 }
 
 __global__ void k_tree_init(DevBfs bfs, uint32_t n, uint32_t* chain, uint8_t* top) {
