@@ -749,8 +749,10 @@ def main():
     roof = roofline_line(stage_roof, dom,
                          {"work_per_launch": stage_work(wc, args.p, best["precision"])[dom][1]})
     # HBM bytes per launch of the dominant kernel, measured by rocprofv3
-    # FETCH_SIZE/WRITE_SIZE runs of this workload (profiles/), when present.
-    tag = f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_{best['precision']}"
+    # FETCH_SIZE/WRITE_SIZE runs of this workload and leaf size (profiles/),
+    # when present.
+    tag = (f"plummer_N{n}_p{args.p}_theta{args.theta}_soft{args.softening}_{best['precision']}"
+           f"_leaf{n_leaf}")
     tf = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
     if os.path.exists(tf):
         t = json.load(open(tf))
