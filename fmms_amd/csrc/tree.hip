@@ -285,8 +285,7 @@ __global__ void __launch_bounds__(TB) k_gather_packed(const double4* __restrict_
                                                      DevBodies b) {
   size_t k = (size_t)blockIdx.x * TB + threadIdx.x;
   if (k >= n) return;
-  // the clamp keeps a failed sort from reading outside the inputs
-  const double4 v = pk[min((size_t)order[k], n - 1)];
+  const double4 v = pk[order[k]];
   b.x[k] = v.x;
   b.y[k] = v.y;
   b.z[k] = v.z;
