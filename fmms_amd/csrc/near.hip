@@ -474,9 +474,17 @@ __global__ void __launch_bounds__(PLAN_T) k_p2p_plan_emit(const uint32_t* __rest
   const uint32_t i = blockIdx.x * (PLAN_T / 64) + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   if (i >= nleaves) return;
+  const uint32_t base = ioff[i];
+  if (ioff[i + 1] - base == 1) {  // one item (the count pass found m = 1): the whole list
+    if (lane == 0) {
+      const uint32_t leaf = leaf_ids[i];
+      items[base] = make_uint4(leaf | P2P_OWN, pptr[leaf], pptr[leaf + 1], P2P_DIRECT);
+    }
+    return;
+  }
   const LeafCut L = leaf_cut(i, leaf_ids, pptr, skeys, smask, ccount, W, lane);
   const uint32_t m = (uint32_t)L.m;
-  const uint32_t base = ioff[i], pb = poff[i];
+  const uint32_t pb = poff[i];
   uint32_t* it = (uint32_t*)(items + base);  // [m][4]: leaf|own, begin, end, slot
   for (uint32_t k = lane; k < m; k += 64) {
     it[4 * k] = L.leaf | (k == 0 ? P2P_OWN : 0u);
