@@ -23,6 +23,7 @@ def runtime():
         _hip.hipFree.argtypes = [ct.c_void_p]
         _hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipStreamSynchronize.argtypes = [ct.c_void_p]
         _hip.hipGetDeviceCount.argtypes = [ct.POINTER(ct.c_int)]
         _hip.hipGetErrorString.restype = ct.c_char_p
     return _hip
@@ -52,6 +53,12 @@ def device_count():
 
 def synchronize():
     _check(runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def synchronize_copies():
+    """Waits for the null stream only (the copies of copy_d2d), not for the
+    library contexts' own (non-blocking) streams, whose work may still run."""
+    _check(runtime().hipStreamSynchronize(None), "hipStreamSynchronize")
 
 
 class DeviceArray:

@@ -1,8 +1,10 @@
 """Serving loop: continuous batching of independent N-body systems on one GPU.
 
 Requests (a new system plus a number of leapfrog steps) arrive over time.
-The server admits them under an HBM budget, computes each newcomer's
-initial forces on its own, and from the next step boundary advances it
+The server admits them under an HBM budget and computes each newcomer's
+initial forces on its own, on a second context and stream, in a worker
+thread, while the batched steps of the active systems go on. From the first
+step boundary after those forces are done it advances the newcomer
 together with every other active system: all active systems form one
 ensemble (fmmh_set_systems) whose state (positions, velocities, masses,
 accelerations, potentials) stays packed in device memory, so one batched
@@ -17,6 +19,7 @@ system per Sim); this is the batched form of its Sim::step loop
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -105,6 +108,11 @@ class Server:
         self.bpb = bytes_per_body(p)
         self.active = []      # admitted systems, in packing order
         self.queue = []       # arrived, not yet admitted
+        # admitted systems whose initial forces are being computed: (slot,
+        # future), in admission order; one worker, as the admission context
+        # takes one call at a time
+        self.pending = []
+        self._pool = ThreadPoolExecutor(max_workers=1)
         self.packed = None    # dict name -> DeviceArray over all active systems
         self.dirty = True     # membership changed since the last packing
         self.step_ms = []     # wall time of each batched step
@@ -115,7 +123,8 @@ class Server:
         return (self.bpb + STATE_BYTES) * n
 
     def _in_use(self):
-        return OVERHEAD_BYTES + sum(self._bytes(s.req.n) for s in self.active)
+        return OVERHEAD_BYTES + sum(self._bytes(s.req.n) for s in self.active) + \
+            sum(self._bytes(s.req.n) for s, _ in self.pending)
 
     def _fits_alone(self, n):
         return OVERHEAD_BYTES + self._bytes(n) <= self.hbm_budget
@@ -144,14 +153,29 @@ class Server:
                    "vel": hip.DeviceArray.from_host(np.ascontiguousarray(req.vel)),
                    "mass": hip.DeviceArray.from_host(np.ascontiguousarray(req.mass)),
                    "acc": hip.DeviceArray((n, 3)), "pot": hip.DeviceArray((n,))}
-            # a(x0) on its own (the same forces it gets inside an ensemble)
-            self.admit_ctx.set_systems([n])
-            self.admit_ctx.accel_device(n, arr["pos"].ptr, arr["mass"].ptr, arr["acc"].ptr,
-                                        arr["pot"].ptr)
+            # a(x0) on its own (the same forces it gets inside an ensemble),
+            # overlapped with the batched steps; it joins once they are done
             req.admitted = now
-            self.active.append(_Slot(req, arr))
-            self.dirty = True
+            self.pending.append((_Slot(req, arr), self._pool.submit(self._initial_forces, n, arr)))
         self.queue = kept
+
+    def _initial_forces(self, n, arr):
+        self.admit_ctx.set_systems([n])
+        self.admit_ctx.accel_device(n, arr["pos"].ptr, arr["mass"].ptr, arr["acc"].ptr,
+                                    arr["pot"].ptr)
+
+    def _join(self):
+        """Moves admitted systems whose initial forces are done (in admission
+        order) into the active set; with nothing active, waits for the first."""
+        while self.pending and (self.pending[0][1].done() or not self.active):
+            slot, fut = self.pending.pop(0)
+            fut.result()  # re-raises a failure of the admission call
+            self.active.append(slot)
+            self.dirty = True
+
+    def close(self):
+        """Waits for admissions in flight and stops the worker thread."""
+        self._pool.shutdown(wait=True)
 
     # ------------------------------------------------------------ packing
     def _pack(self):
@@ -169,7 +193,7 @@ class Server:
                     src = self.packed[name].ptr + s.offset * w * 8
                 hip.copy_d2d(new[name].ptr + off * w * 8, src, s.req.n * w * 8)
             off += s.req.n
-        hip.synchronize()
+        hip.synchronize_copies()
         off = 0
         for s in self.active:
             if s.offset is None:
@@ -195,7 +219,7 @@ class Server:
             a = hip.DeviceArray(shape)
             hip.copy_d2d(a.ptr, self.packed[name].ptr + slot.offset * w * 8, slot.req.n * w * 8)
             out.append(a)
-        hip.synchronize()
+        hip.synchronize_copies()
         res = tuple(a.download() for a in out)
         for a in out:
             a.free()
@@ -203,11 +227,14 @@ class Server:
 
     # --------------------------------------------------------------- step
     def step(self, now):
-        """One step boundary: admit arrivals (initial forces, packing), one
-        batched KDK step of every active system, retire finished systems.
-        Returns the number of bodies stepped."""
+        """One step boundary: admit arrivals (their initial forces start on
+        the admission context), join the admitted systems whose initial
+        forces are done (packing), one batched KDK step of every active
+        system, retire finished systems. Returns the number of bodies
+        stepped."""
         t0 = time.perf_counter()
         self._admit(now)
+        self._join()
         if not self.active:
             return 0
         if self.dirty:
@@ -240,7 +267,7 @@ class Server:
         pending = sorted(requests, key=lambda r: r.arrival)
         t_start = time.perf_counter()
         i = 0
-        while i < len(pending) or self.queue or self.active:
+        while i < len(pending) or self.queue or self.active or self.pending:
             now = time.perf_counter() - t_start
             if max_seconds is not None and now > max_seconds:
                 break
@@ -250,7 +277,7 @@ class Server:
                 except FmmError:
                     pass  # marked rejected; it never runs
                 i += 1
-            if not self.queue and not self.active:
+            if not self.queue and not self.active and not self.pending:
                 time.sleep(max(0.0, min(pending[i].arrival - now, 0.01)))
                 continue
             self.step(now)
